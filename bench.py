@@ -1,0 +1,137 @@
+"""Benchmark: node-steps/s of the HIP PeriDEM stepper on the 3D compression
+workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
+moving top wall, gravity), one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  `value` is node-steps/s with the state resident in
+HBM; the timed region is K steps enqueued through pg_step, bracketed by HIP
+events on the stepper's own stream.  `roofline` prices the fused step
+kernel against HBM bandwidth with the algorithmic bytes of one step,
+N*(8*(8d+1)+16) + 5*M (N nodes, M directed bond slots, d=3).
+`cpu_baseline` times the FP64 CPU restatement of the reference algorithm
+(the parity oracle) on a few steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_nodes, n_slots, dim):
+    return n_nodes * (8 * (8 * dim + 1) + 16) + 5 * n_slots
+
+
+def cpu_baseline(setup, params, threads, timed_steps=2):
+    import oracle
+    o = oracle.Oracle(setup, params, threads=threads)
+    o.step(1)  # warm-up
+    t0 = time.perf_counter()
+    o.step(timed_steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": setup.n_nodes * timed_steps / dt,
+        "unit": "node-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{timed_steps} steps after 1 warm-up step of the same 3D workload "
+                  f"({setup.n_nodes} nodes), {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(see DESIGN.md for how the stepper would shard).", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: multi-rank launch is not supported; run with one process.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from perigrain_amd import Stepper, setups
+
+    t0 = time.perf_counter()
+    setup, params = setups.compression_3d(seed=args.seed)
+    print(f"bench: workload built in {time.perf_counter() - t0:.1f} s: {setup.n_nodes} nodes, "
+          f"{setup.n_slots()} directed bonds", file=sys.stderr, flush=True)
+    dim = setup.dim
+    with Stepper(setup, params, device=0) as g:
+        g.step(args.warmup)
+        g.sync()
+        st0 = g.stats()
+        g.event(0)
+        g.step(args.steps)
+        g.event(1)
+        ms = g.elapsed_ms(0, 1)
+        g.sync()
+        st1 = g.stats()
+    n_nodes, n_slots = setup.n_nodes, setup.n_slots()
+    ms_per_step = ms / args.steps
+    value = n_nodes * args.steps / (ms / 1e3)
+    bytes_step = algorithmic_bytes(n_nodes, n_slots, dim)
+    achieved = bytes_step / (ms_per_step / 1e3) / 1e9
+    print(f"bench: {args.steps} steps in {ms:.2f} ms ({ms_per_step * 1e3:.2f} us/step), "
+          f"list rebuilds {st1['rebuilds'] - st0['rebuilds']}, "
+          f"skipped steps {st1['skipped_steps'] - st0['skipped_steps']}",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(setup, params, threads)
+
+    line = {
+        "metric": "node-steps/s at 256k-node 3D",
+        "value": value,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered-lattice spheres)",
+        "config": {
+            "workload": "3d_compression_64_spheres",
+            "nodes": n_nodes,
+            "directed_bonds": n_slots,
+            "grains": setup.n_grains,
+            "dt": params.dt,
+            "fracture": bool(params.enable_fracture),
+            "top_wall_speed": float(params.wall_speed[5]),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_step": bytes_step,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * perigrain.h -- C ABI of libpg, the MI355X (gfx950) PeriDEM time stepper.
+ *
+ * The reference simulator exposes no FFI: its hot path is the C++ template
+ *   template<unsigned dim> void run_timeloop(vector<ParticleN<dim>>& PArr,
+ *        Timeloop TL, Contact CN, RectWall<dim> Wall)
+ *   (src/particle/timeloop.h:500-502), driven by run/simulate2d.cpp:31-94.
+ * libpg replaces that call.  The host (simulate2d/simulate3d drop-ins, the
+ * Python binding) flattens what load_particles/load_contact/load_wall
+ * (src/read/load_hdf5.h:41-201) produce, after the apply_config overrides
+ * (src/particle/timeloop.h:174-202, src/particle/contact.h:85-108,254-301),
+ * into a pg_setup + pg_params pair, and drives the loop through pg_step.
+ *
+ * Conventions
+ *  - All entry points return PG_OK (0) or a PG_ERR_* code; no C++ exception
+ *    crosses the ABI.  pg_last_error() returns a thread-local message.
+ *  - Arrays passed in are caller-owned host memory; they are copied.
+ *  - Node arrays are row-major [node][component], nodes of grain g occupying
+ *    [node_offset[g], node_offset[g+1]) (the reference's P_%05u order).
+ *  - All physics is IEEE binary64.  Indices are uint32 (grain-local in the
+ *    setup, as in the reference's Connectivity datasets).
+ *  - One host thread per handle; handles are independent.
+ */
+#ifndef PERIGRAIN_H
+#define PERIGRAIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_ERR_INVALID 1 /* bad argument or inconsistent setup */
+#define PG_ERR_DEVICE 2  /* HIP runtime error */
+#define PG_ERR_ALLOC 3   /* device or host allocation failed */
+#define PG_ERR_STATE 4   /* call not valid in the handle's state */
+#define PG_ERR_IO 5      /* file I/O (HDF5) error */
+
+/* Wall sides.  2D order follows RectWall::lrtb() (contact.h:139-143):
+ * left, right, top, bottom.  3D order follows contact.h:145-150:
+ * x_min, y_min, z_min, x_max, y_max, z_max. */
+#define PG_MAX_SIDES 6
+
+/* One simulation's initial state (what data/hdf5/all.h5 holds, after the
+ * config overrides of Contact::apply_config and RectWall::apply_config). */
+typedef struct pg_setup {
+  int32_t dim;                /* 2 or 3 */
+  int32_t n_grains;           /* total_particles_univ */
+  const int64_t *node_offset; /* [n_grains+1] */
+  const double *pos;          /* [N*dim]  P_xxxxx/Pos */
+  const double *vol;          /* [N]      P_xxxxx/Vol */
+  const double *disp;         /* [N*dim]  P_xxxxx/disp */
+  const double *vel;          /* [N*dim]  P_xxxxx/vel */
+  const double *acc;          /* [N*dim]  P_xxxxx/acc */
+  const double *extforce;     /* [N*dim]  P_xxxxx/extforce (force density) */
+  const int64_t *conn_offset; /* [n_grains+1] rows of Connectivity per grain */
+  const uint32_t *conn;       /* [rows*2] grain-local (i,j) pairs, file order */
+  const int64_t *bdry_offset; /* [n_grains+1] */
+  const uint32_t *bdry_nodes; /* grain-local, file order (nl_bdry_only) */
+  const int64_t *clamped_offset; /* [n_grains+1] */
+  const uint32_t *clamped_nodes; /* grain-local */
+  const double *delta;        /* [n_grains] */
+  const double *rho;          /* [n_grains] */
+  const double *cnot;         /* [n_grains] PMB micromodulus c */
+  const double *snot;         /* [n_grains] critical stretch s0 */
+  const int32_t *movable;     /* [n_grains] */
+  const int32_t *breakable;   /* [n_grains] */
+  const int32_t *stoppable;   /* [n_grains] */
+  const uint32_t *torque_axis; /* [n_grains] or NULL (read only if enable_torque) */
+  const double *torque_val;   /* [n_grains] or NULL */
+  double contact_radius;      /* Contact::contact_rad */
+  double normal_stiffness;    /* Contact::normal_stiffness (after override) */
+  double damping_ratio;       /* after override */
+  double friction_coefficient;/* after override */
+  int32_t allow_wall;
+  double wall[PG_MAX_SIDES];  /* lrtb() order, after RectWall::apply_config */
+} pg_setup;
+
+/* Time-loop and contact toggles (Timeloop/Contact/RectWall after
+ * apply_config; keys of config/main.conf, src/read/read_config.h:298-446). */
+typedef struct pg_params {
+  double dt;
+  int32_t enable_fracture;
+  int32_t nl_bdry_only;
+  int32_t allow_damping;
+  int32_t allow_friction;
+  int32_t self_contact;
+  int32_t gradient_extforce;
+  uint32_t extforce_maxstep;
+  int32_t enable_torque;
+  int32_t set_movable_index;     /* -1 = unused */
+  int32_t set_movable_timestep;
+  int32_t set_stoppable_index;   /* -1 = unused */
+  int32_t set_stoppable_timestep;
+  int32_t reset_partzero_y;
+  uint32_t reset_partzero_y_timestep;
+  double wheel_rad;
+  double wall_speed[PG_MAX_SIDES]; /* same order as pg_setup.wall */
+} pg_params;
+
+/* Fields readable with pg_read_nodes. */
+#define PG_FIELD_CURRPOS 0 /* CurrPos of the last completed step t */
+#define PG_FIELD_VEL 1     /* vel after step t */
+#define PG_FIELD_ACC 2     /* acc after step t */
+#define PG_FIELD_FORCE 3   /* force density of step t */
+#define PG_FIELD_DISP 4    /* disp used by step t (CurrPos = pos + disp) */
+
+typedef struct pg_handle pg_handle;
+
+/* Thread-local description of the last error on this thread. */
+const char *pg_last_error(void);
+
+/* Library/device information: writes a short NUL-terminated string. */
+int pg_version(char *buf, size_t cap);
+
+/* Number of HIP devices visible to the library. */
+int pg_device_count(int *n);
+
+/* Diagnostics (no reference counterpart): q[i] = a[i] / b[i] and
+ * r[i] = sqrt(a[i]) computed on `device` with the operations the step
+ * kernels use, so callers can check their rounding against the host's. */
+int pg_check_arith(int device, uint64_t n, const double *a, const double *b, double *q,
+                   double *r);
+
+/* Create a stepper for one simulation on HIP device `device`.
+ * Replaces the state set-up of run_timeloop (timeloop.h:504-557; gen_xi at
+ * particle2.h:236-252 and conn2NArr at load_hdf5.h:28-38) without the file
+ * I/O.  The step counter starts at t=0 (run_timeloop's loop starts at t=1
+ * even on resume, timeloop.h:608). */
+int pg_create(const pg_setup *setup, const pg_params *params, int device,
+              pg_handle **out);
+
+/* Batched form: n independent simulations packed into one handle and
+ * advanced together (one fused launch per step).  Each instance keeps its
+ * own walls, contact constants and toggles; instances never interact.
+ * All instances must share dim and dt. No reference counterpart. */
+int pg_batch_create(const pg_setup *setups, const pg_params *params, int n,
+                    int device, pg_handle **out);
+
+int pg_destroy(pg_handle *h);
+
+/* Advance every instance by nsteps time steps (the body of the loop at
+ * timeloop.h:608-878 without the snapshot write).  Force density and the
+ * wall reaction are recorded for the last of these steps.  Stream-ordered:
+ * returns once the work is enqueued; the read functions synchronise. */
+int pg_step(pg_handle *h, uint32_t nsteps);
+
+/* Block until all enqueued work on the handle's stream is done. */
+int pg_sync(pg_handle *h);
+
+/* Index t of the last completed step (0 before the first pg_step). */
+int pg_current_step(pg_handle *h, uint32_t *t);
+
+/* Sizes of instance `inst`: node count, grain count, directed bond slots. */
+int pg_sizes(pg_handle *h, int inst, uint64_t *n_nodes, uint64_t *n_grains,
+             uint64_t *n_slots);
+
+/* Copy a node field of instance `inst` ([N*dim] doubles) to host memory. */
+int pg_read_nodes(pg_handle *h, int inst, int field, double *dst);
+
+/* Wall positions after step t's motion (wall_xxxxx.h5:wall_info) and the
+ * reaction of step t (wall_xxxxx.h5:reaction, get_reaction() order,
+ * contact.h:185-202), [PG_MAX_SIDES] and [PG_MAX_SIDES*dim] doubles; only
+ * the first 4 / 4*dim entries are meaningful in 2D. */
+int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction);
+
+/* Per directed bond slot, in the reference's NbdArr order (grain, node,
+ * position in the node's row as built by conn2NArr): 1 = intact, 0 = broken.
+ * [n_slots] bytes. */
+int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst);
+
+/* Intact bonds of grain `grain` as NbdArr2conn (timeloop.h:25-49) writes
+ * them: rows (i,q) with i<q in adjacency order.  Call with dst=NULL to get
+ * the row count in *rows. */
+int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst,
+                 uint64_t cap_rows, uint64_t *rows);
+
+/* Node-node contact pairs that produced a contact force in step t, as
+ * (node i, partner j) pairs of instance-global node indices, ordered by i
+ * and then by the order in which the reference accumulates them
+ * (contact.h:675-811 called from timeloop.h:709-747).  dst=NULL returns the
+ * count only. */
+int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs,
+                     uint64_t *pairs);
+
+/* Stream (hipStream_t) the handle enqueues its kernels on. */
+int pg_stream(pg_handle *h, void **stream);
+
+/* Statistics of the contact neighbour list: rebuild count, entries. */
+int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries,
+             uint64_t *skipped_steps);
+
+/* Tuning knobs (used by the autotuner).  skin_factor: Verlet skin as a
+ * multiple of the contact radius; block: threads per block of the step
+ * kernel (64..1024, multiple of 64).  Takes effect at the next rebuild. */
+int pg_set_tuning(pg_handle *h, double skin_factor, int block);
+
+/* Capture chunks of steps into hipGraphs (1) or launch each step (0). */
+int pg_set_graph(pg_handle *h, int use_graph);
+
+/* HIP events on the handle's stream (slots 0..7), for timing by callers:
+ * record an event, then read the elapsed milliseconds between two slots
+ * (waits for the later one). */
+int pg_event_record(pg_handle *h, int slot);
+int pg_event_elapsed(pg_handle *h, int a, int b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PERIGRAIN_H */

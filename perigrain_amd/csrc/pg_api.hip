@@ -1,0 +1,953 @@
+// pg_api.hip -- host implementation of the libpg C ABI (include/perigrain.h).
+//
+// A handle owns the device state of one or more packed instances and drives
+// one fused step kernel per time step on its own HIP stream.  The contact
+// neighbour list is a Verlet list with a skin: each step kernel checks how
+// far contact nodes moved since the list was built and, the first time one
+// moved more than half the skin, records that step; that kernel and all
+// later ones of the same batch return at once without touching state.  The
+// host checks the record after each batch of launches, rebuilds the list
+// from the positions of the first unexecuted step and relaunches from there,
+// so no step ever runs with a stale list.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "perigrain.h"
+#include "pg_internal.h"
+#include "pg_kernels.h"
+
+using namespace pg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(PG_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct InstHost {
+  uint32_t node_begin, node_end, grain_begin, grain_end;
+  uint64_t slot_begin, slot_end;
+  uint32_t ckey_begin, ckey_end;
+  double Rc, skin;
+  bool contact;
+  int nsides;
+};
+
+} // namespace
+
+struct pg_handle {
+  int dim = 0, device = 0, block = 256;
+  hipStream_t stream = nullptr;
+  uint32_t N = 0, G = 0, I = 0, S = 0, CK = 0;
+  uint64_t E = 0;
+  uint32_t t = 0;
+  uint32_t check_every = 64;
+  double skin_factor = 0.5;
+  bool any_torque = false, any_contact = false;
+  std::vector<InstHost> inst;
+  std::vector<InstDev> inst_dev;
+  std::vector<uint32_t> deg, slice_base, slice_width, bcol, grain_node_begin;
+  std::vector<uint64_t> slot0;
+  std::vector<void *> allocs;
+  // device
+  NodeRec *rec[2] = {nullptr, nullptr};
+  double *vel[2] = {nullptr, nullptr};
+  double *acc = nullptr, *ext = nullptr, *xb = nullptr, *gmean = nullptr;
+  uint32_t *node_grain = nullptr, *ckey = nullptr;
+  uint8_t *node_info = nullptr;
+  GrainDev *grains = nullptr;
+  InstDev *insts = nullptr;
+  uint32_t *d_slice_base = nullptr, *d_slice_width = nullptr, *d_bcol = nullptr;
+  uint8_t *bstat = nullptr;
+  uint32_t *cl_ptr = nullptr, *cl_j = nullptr;
+  int32_t *cl_slot = nullptr;
+  size_t cl_cap = 0;
+  unsigned long long *bbox[3] = {nullptr, nullptr, nullptr};
+  double *wallpos[2] = {nullptr, nullptr};
+  uint32_t *violation = nullptr;
+  double *force_out = nullptr, *wterm = nullptr, *reaction = nullptr;
+  uint8_t *wflag = nullptr;
+  // list build scratch
+  GridDev *grids = nullptr;
+  unsigned long long *ibox = nullptr;
+  uint32_t *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_cur = nullptr,
+           *cell_nodes = nullptr, *cl_cnt = nullptr;
+  size_t cell_cap = 0;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  uint32_t *h_pin = nullptr;
+  hipEvent_t ev[8] = {};
+  uint64_t rebuilds = 0, skipped = 0, list_entries = 0;
+  StepArgs A{};
+};
+
+namespace {
+
+template <class T> int dalloc(pg_handle *h, T *&p, size_t n) {
+  if (n == 0) n = 1;
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(PG_ERR_ALLOC, std::string("hipMalloc ") + std::to_string(n * sizeof(T)) +
+                                  " bytes: " + hipGetErrorString(e));
+  h->allocs.push_back(q);
+  p = static_cast<T *>(q);
+  return PG_OK;
+}
+
+template <class T> int dfree(pg_handle *h, T *&p) {
+  if (!p) return PG_OK;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), (void *)p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  hipError_t e = hipFree(p);
+  p = nullptr;
+  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipFree: ") + hipGetErrorString(e));
+  return PG_OK;
+}
+
+template <class T> int h2d(pg_handle *h, T *dst, const T *src, size_t n) {
+  if (n == 0) return PG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  return PG_OK;
+}
+template <class T> int d2h(pg_handle *h, T *dst, const T *src, size_t n) {
+  if (n == 0) return PG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+#define RC(x)                  \
+  do {                         \
+    int rc_ = (x);             \
+    if (rc_ != PG_OK) return rc_; \
+  } while (0)
+
+void fill_args(pg_handle *h) {
+  StepArgs &A = h->A;
+  A.rec[0] = h->rec[0];
+  A.rec[1] = h->rec[1];
+  A.vel[0] = h->vel[0];
+  A.vel[1] = h->vel[1];
+  A.acc = h->acc;
+  A.ext = h->ext;
+  A.node_grain = h->node_grain;
+  A.node_info = h->node_info;
+  A.ckey = h->ckey;
+  A.grains = h->grains;
+  A.insts = h->insts;
+  A.slice_base = h->d_slice_base;
+  A.slice_width = h->d_slice_width;
+  A.bcol = h->d_bcol;
+  A.bstat = h->bstat;
+  A.cl_ptr = h->cl_ptr;
+  A.cl_j = h->cl_j;
+  A.cl_slot = h->cl_slot;
+  A.xb = h->xb;
+  for (int k = 0; k < 3; ++k) A.bbox[k] = h->bbox[k];
+  A.wallpos[0] = h->wallpos[0];
+  A.wallpos[1] = h->wallpos[1];
+  A.gmean = h->gmean;
+  A.violation = h->violation;
+  A.force_out = h->force_out;
+  A.wterm = h->wterm;
+  A.wflag = h->wflag;
+  A.n_nodes = h->N;
+  A.n_grains = h->G;
+  A.n_insts = h->I;
+  A.any_torque = h->any_torque ? 1 : 0;
+}
+
+int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
+  size_t need = 0;
+  HIPCHK(exclusive_scan_u32(in, out, n, nullptr, &need, h->stream));
+  if (need > h->scan_tmp_bytes) {
+    if (h->scan_tmp) {
+      auto it = std::find(h->allocs.begin(), h->allocs.end(), h->scan_tmp);
+      if (it != h->allocs.end()) h->allocs.erase(it);
+      HIPCHK(hipFree(h->scan_tmp));
+      h->scan_tmp = nullptr;
+    }
+    char *p = nullptr;
+    RC(dalloc(h, p, need));
+    h->scan_tmp = p;
+    h->scan_tmp_bytes = need;
+  }
+  size_t bytes = h->scan_tmp_bytes;
+  HIPCHK(exclusive_scan_u32(in, out, n, h->scan_tmp, &bytes, h->stream));
+  return PG_OK;
+}
+
+// Build the contact Verlet list from the positions of step ts
+// (u^ts in rec[ts & 1]).
+int build_list(pg_handle *h, uint32_t ts) {
+  const int D = h->dim;
+  if (!h->any_contact) {
+    HIPCHK(hipMemsetAsync(h->cl_ptr, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
+    return PG_OK;
+  }
+  // 1. per-instance box of contact nodes
+  HIPCHK(launch_fill_box(h->ibox, (size_t)h->I * 2 * D, D, h->stream));
+  HIPCHK(launch_inst_box(D, h->A, ts, h->ibox, h->stream));
+  std::vector<unsigned long long> kb((size_t)h->I * 2 * D);
+  RC(d2h(h, kb.data(), h->ibox, kb.size()));
+  // 2. per-instance grids
+  std::vector<GridDev> gd(h->I);
+  uint64_t cells = 0;
+  for (uint32_t s = 0; s < h->I; ++s) {
+    const InstHost &ih = h->inst[s];
+    GridDev &g = gd[s];
+    std::memset(&g, 0, sizeof(g));
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool empty = !ih.contact;
+    for (int k = 0; k < D; ++k) {
+      lo[k] = dkey_inv(kb[(size_t)s * 2 * D + k]);
+      hi[k] = dkey_inv(kb[(size_t)s * 2 * D + D + k]);
+      if (!(lo[k] <= hi[k]) || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) empty = true;
+    }
+    const double rs = (ih.Rc + ih.skin) * (1.0 + 1e-9);
+    g.r2 = rs * rs;
+    double L = rs;
+    const uint32_t ncontact = ih.node_end - ih.node_begin;
+    const double cap = std::max<double>(4096.0, 4.0 * ncontact);
+    int dims[3] = {1, 1, 1};
+    if (!empty && rs > 0) {
+      for (int it = 0; it < 64; ++it) {
+        double tot = 1;
+        for (int k = 0; k < D; ++k) {
+          double n = std::floor((hi[k] - lo[k]) / L) + 1;
+          if (n < 1) n = 1;
+          dims[k] = (int)std::min(n, 1e9);
+          tot *= n;
+        }
+        if (tot <= cap) break;
+        L *= (D == 3) ? std::cbrt(tot / cap) * 1.01 : std::sqrt(tot / cap) * 1.01;
+      }
+    }
+    for (int k = 0; k < 3; ++k) g.origin[k] = (k < D && !empty) ? lo[k] : 0.0;
+    g.inv_cell = (L > 0 && std::isfinite(L)) ? 1.0 / L : 0.0;
+    for (int k = 0; k < 3; ++k) g.dims[k] = k < D ? dims[k] : 1;
+    g.cell_begin = (uint32_t)cells;
+    cells += (uint64_t)g.dims[0] * g.dims[1] * g.dims[2];
+  }
+  if (cells + 1 >= 0xFFFFFFFFull) return fail(PG_ERR_INVALID, "cell grid too large");
+  RC(h2d(h, h->grids, gd.data(), gd.size()));
+  if (cells + 1 > h->cell_cap) {
+    RC(dfree(h, h->cell_cnt));
+    RC(dfree(h, h->cell_start));
+    RC(dfree(h, h->cell_cur));
+    h->cell_cap = (size_t)((cells + 1) * 1.25) + 64;
+    RC(dalloc(h, h->cell_cnt, h->cell_cap));
+    RC(dalloc(h, h->cell_start, h->cell_cap));
+    RC(dalloc(h, h->cell_cur, h->cell_cap));
+  }
+  // 3. counting sort of contact nodes into cells
+  HIPCHK(hipMemsetAsync(h->cell_cnt, 0, (cells + 1) * sizeof(uint32_t), h->stream));
+  HIPCHK(launch_cell_count(D, h->A, ts, h->grids, h->cell_of, h->cell_cnt, h->stream));
+  RC(scan(h, h->cell_cnt, h->cell_start, cells + 1));
+  HIPCHK(hipMemsetAsync(h->cell_cur, 0, (cells + 1) * sizeof(uint32_t), h->stream));
+  HIPCHK(launch_cell_fill(h->N, h->cell_of, h->cell_start, h->cell_cur, h->cell_nodes, h->stream));
+  // 4. count, scan, fill the per-node lists
+  HIPCHK(hipMemsetAsync(h->cl_cnt, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
+  HIPCHK(launch_list(D, false, h->A, ts, h->grids, h->cell_start, h->cell_nodes, h->cl_cnt,
+                     nullptr, nullptr, nullptr, nullptr, h->stream));
+  RC(scan(h, h->cl_cnt, h->cl_ptr, (size_t)h->N + 1));
+  uint32_t total = 0;
+  RC(d2h(h, &total, h->cl_ptr + h->N, 1));
+  if (total > h->cl_cap) {
+    RC(dfree(h, h->cl_j));
+    RC(dfree(h, h->cl_slot));
+    h->cl_cap = (size_t)(total * 1.5) + 1024;
+    RC(dalloc(h, h->cl_j, h->cl_cap));
+    RC(dalloc(h, h->cl_slot, h->cl_cap));
+    fill_args(h);
+  }
+  HIPCHK(launch_list(D, true, h->A, ts, h->grids, h->cell_start, h->cell_nodes, nullptr,
+                     h->cl_ptr, h->cl_j, h->cl_slot, h->xb, h->stream));
+  HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
+  h->list_entries = total;
+  h->rebuilds += 1;
+  return PG_OK;
+}
+
+int check_setup(const pg_setup *s, const pg_params *p, int dim_expected) {
+  if (!s || !p) return fail(PG_ERR_INVALID, "null setup or params");
+  if (s->dim != 2 && s->dim != 3) return fail(PG_ERR_INVALID, "dim must be 2 or 3");
+  if (dim_expected && s->dim != dim_expected)
+    return fail(PG_ERR_INVALID, "all instances must share one dimension");
+  if (s->n_grains < 1) return fail(PG_ERR_INVALID, "n_grains must be >= 1");
+  if (!s->node_offset || !s->pos || !s->vol || !s->disp || !s->vel || !s->acc ||
+      !s->extforce || !s->conn_offset || !s->bdry_offset || !s->clamped_offset || !s->delta ||
+      !s->rho || !s->cnot || !s->snot || !s->movable || !s->breakable || !s->stoppable)
+    return fail(PG_ERR_INVALID, "missing setup array");
+  if (s->node_offset[0] != 0 || s->conn_offset[0] != 0 || s->bdry_offset[0] != 0 ||
+      s->clamped_offset[0] != 0)
+    return fail(PG_ERR_INVALID, "offset arrays must start at 0");
+  for (int g = 0; g < s->n_grains; ++g) {
+    if (s->node_offset[g + 1] <= s->node_offset[g])
+      return fail(PG_ERR_INVALID, "grain " + std::to_string(g) + " has no nodes");
+    if (s->conn_offset[g + 1] < s->conn_offset[g] || s->bdry_offset[g + 1] < s->bdry_offset[g] ||
+        s->clamped_offset[g + 1] < s->clamped_offset[g])
+      return fail(PG_ERR_INVALID, "offset arrays must be non-decreasing");
+    if (p->enable_torque) {
+      if (!s->torque_axis || !s->torque_val)
+        return fail(PG_ERR_INVALID, "enable_torque needs torque_axis and torque_val");
+      if (s->torque_axis[g] > 2 || (s->dim == 2 && s->torque_axis[g] != 2))
+        return fail(PG_ERR_INVALID, "torque_axis must be 2 in 2D and < 3 in 3D");
+    }
+  }
+  if (s->node_offset[s->n_grains] >= 0xFFFFFF00ll)
+    return fail(PG_ERR_INVALID, "too many nodes");
+  if (!(p->dt > 0)) return fail(PG_ERR_INVALID, "dt must be > 0");
+  if (p->reset_partzero_y)
+    return fail(PG_ERR_INVALID, "reset_partzero_y is not supported by the HIP stepper");
+  auto chk_idx = [&](int32_t idx, int32_t ts) -> bool {
+    return idx == -1 || ts <= 0 || (idx >= 0 && idx < s->n_grains);
+  };
+  if (!chk_idx(p->set_movable_index, p->set_movable_timestep) ||
+      !chk_idx(p->set_stoppable_index, p->set_stoppable_timestep))
+    return fail(PG_ERR_INVALID, "set_movable_index/set_stoppable_index out of range");
+  return PG_OK;
+}
+
+int create_impl(const pg_setup *setups, const pg_params *params, int n, int device,
+                pg_handle *h) {
+  if (n < 1) return fail(PG_ERR_INVALID, "need at least one instance");
+  const int D = setups[0].dim;
+  for (int s = 0; s < n; ++s) RC(check_setup(&setups[s], &params[s], D));
+  h->dim = D;
+  h->device = device;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
+
+  // ---- sizes and per-instance ranges
+  uint64_t N = 0, G = 0, CK = 0, slots = 0;
+  h->inst.resize(n);
+  for (int s = 0; s < n; ++s) {
+    const pg_setup &st = setups[s];
+    InstHost &ih = h->inst[s];
+    ih.node_begin = (uint32_t)N;
+    ih.grain_begin = (uint32_t)G;
+    N += (uint64_t)st.node_offset[st.n_grains];
+    G += (uint64_t)st.n_grains;
+    ih.node_end = (uint32_t)N;
+    ih.grain_end = (uint32_t)G;
+    ih.Rc = st.contact_radius;
+    ih.contact = st.contact_radius > 0;
+    ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
+    ih.nsides = D == 2 ? 4 : 6;
+  }
+  if (N >= 0xFFFFFF00ull) return fail(PG_ERR_INVALID, "too many nodes in batch");
+  h->N = (uint32_t)N;
+  h->G = (uint32_t)G;
+  h->I = (uint32_t)n;
+
+  // ---- host staging of node data
+  std::vector<NodeRec> rec(N);
+  std::vector<double> vel(N * D), acc(N * D), ext(N * D);
+  std::vector<uint32_t> node_grain(N), ckey(N, 0xFFFFFFFFu);
+  std::vector<uint8_t> node_info(N, 0);
+  std::vector<GrainDev> grains(G);
+  h->inst_dev.resize(n);
+  h->deg.assign(N, 0);
+  h->grain_node_begin.resize(G + 1);
+  std::vector<uint64_t> row_ptr(N + 1, 0);
+
+  for (int s = 0; s < n; ++s) {
+    const pg_setup &st = setups[s];
+    const pg_params &pp = params[s];
+    InstHost &ih = h->inst[s];
+    const bool bdry_only = pp.nl_bdry_only != 0;
+    // nodes
+    for (int g = 0; g < st.n_grains; ++g) {
+      const uint32_t gg = ih.grain_begin + g;
+      const int64_t a = st.node_offset[g], b = st.node_offset[g + 1];
+      h->grain_node_begin[gg] = ih.node_begin + (uint32_t)a;
+      for (int64_t q = a; q < b; ++q) {
+        const uint64_t i = ih.node_begin + q;
+        NodeRec &r = rec[i];
+        std::memset(&r, 0, sizeof(r));
+        for (int k = 0; k < D; ++k) {
+          r.X[k] = st.pos[q * D + k];
+          r.u[k] = st.disp[q * D + k];
+          vel[i * D + k] = st.vel[q * D + k];
+          acc[i * D + k] = st.acc[q * D + k];
+          ext[i * D + k] = st.extforce[q * D + k];
+        }
+        r.X[D] = st.vol[q];
+        node_grain[i] = gg;
+        if (!bdry_only && ih.contact) node_info[i] |= NI_CONTACT;
+      }
+      for (int64_t c = st.clamped_offset[g]; c < st.clamped_offset[g + 1]; ++c) {
+        const uint32_t cn = st.clamped_nodes[c];
+        if (cn >= (uint64_t)(b - a)) return fail(PG_ERR_INVALID, "clamped node out of range");
+        node_info[ih.node_begin + a + cn] |= NI_CLAMPED;
+      }
+      GrainDev &gd = grains[gg];
+      std::memset(&gd, 0, sizeof(gd));
+      gd.rho = st.rho[g];
+      gd.cnot = st.cnot[g];
+      gd.snot = st.snot[g];
+      gd.yshift = 0.0;
+      gd.flags = (st.movable[g] ? GF_MOVABLE : 0) | (st.breakable[g] ? GF_BREAKABLE : 0) |
+                 (st.stoppable[g] ? GF_STOPPABLE : 0);
+      gd.inst = s;
+      gd.node_begin = ih.node_begin + (uint32_t)a;
+      gd.node_end = ih.node_begin + (uint32_t)b;
+      gd.torque_axis = (pp.enable_torque && st.torque_axis) ? st.torque_axis[g] : 2;
+      gd.torque_val = (pp.enable_torque && st.torque_val) ? st.torque_val[g] : 0.0;
+    }
+    // contact order keys: node order, or boundary-list order (contact.h:672-681)
+    ih.ckey_begin = (uint32_t)CK;
+    if (!bdry_only) {
+      for (uint32_t i = ih.node_begin; i < ih.node_end; ++i) ckey[i] = (uint32_t)(CK++);
+    } else {
+      for (int g = 0; g < st.n_grains; ++g) {
+        const int64_t a = st.node_offset[g], nn = st.node_offset[g + 1] - a;
+        for (int64_t c = st.bdry_offset[g]; c < st.bdry_offset[g + 1]; ++c) {
+          const uint32_t bn = st.bdry_nodes[c];
+          if (bn >= (uint64_t)nn) return fail(PG_ERR_INVALID, "boundary node out of range");
+          const uint64_t i = ih.node_begin + a + bn;
+          if (ckey[i] != 0xFFFFFFFFu)
+            return fail(PG_ERR_INVALID, "duplicate entry in bdry_nodes of grain " + std::to_string(g));
+          ckey[i] = (uint32_t)(CK++);
+          if (ih.contact) node_info[i] |= NI_CONTACT;
+        }
+      }
+    }
+    ih.ckey_end = (uint32_t)CK;
+    // degrees (conn2NArr, load_hdf5.h:28-38)
+    for (int g = 0; g < st.n_grains; ++g) {
+      const int64_t a = st.node_offset[g], nn = st.node_offset[g + 1] - a;
+      for (int64_t r = st.conn_offset[g]; r < st.conn_offset[g + 1]; ++r) {
+        const uint32_t ci = st.conn[2 * r], cj = st.conn[2 * r + 1];
+        if (ci >= (uint64_t)nn || cj >= (uint64_t)nn)
+          return fail(PG_ERR_INVALID, "Connectivity index out of range in grain " + std::to_string(g));
+        if (ci == cj) return fail(PG_ERR_INVALID, "Connectivity has a self bond");
+        h->deg[ih.node_begin + a + ci]++;
+        h->deg[ih.node_begin + a + cj]++;
+      }
+    }
+    // instance constants
+    InstDev &id = h->inst_dev[s];
+    std::memset(&id, 0, sizeof(id));
+    id.dt = pp.dt;
+    id.h = (pp.dt * pp.dt * 0.5);
+    id.half_dt = (0.5 * pp.dt);
+    id.Rc = st.contact_radius;
+    id.Kn = st.normal_stiffness;
+    id.zeta = st.damping_ratio;
+    id.mu = st.friction_coefficient;
+    for (int k = 0; k < 6; ++k) id.speed[k] = pp.wall_speed[k];
+    id.flags = (pp.enable_fracture ? IF_FRACTURE : 0) | (bdry_only ? IF_BDRY_ONLY : 0) |
+               (pp.allow_damping ? IF_DAMPING : 0) | (pp.allow_friction ? IF_FRICTION : 0) |
+               (pp.self_contact ? IF_SELF_CONTACT : 0) | (st.allow_wall ? IF_WALL : 0) |
+               (pp.gradient_extforce ? IF_GRAD_EXT : 0) | (pp.enable_torque ? IF_TORQUE : 0);
+    id.extforce_maxstep = pp.extforce_maxstep;
+    id.set_movable_grain = (pp.set_movable_index >= 0 && pp.set_movable_timestep > 0)
+                               ? (int32_t)(ih.grain_begin + pp.set_movable_index) : -1;
+    id.set_movable_step = (uint32_t)std::max(pp.set_movable_timestep, 0);
+    id.set_stoppable_grain = (pp.set_stoppable_index >= 0 && pp.set_stoppable_timestep > 0)
+                                 ? (int32_t)(ih.grain_begin + pp.set_stoppable_index) : -1;
+    id.set_stoppable_step = (uint32_t)std::max(pp.set_stoppable_timestep, 0);
+    id.grain_begin = ih.grain_begin;
+    id.grain_end = ih.grain_end;
+    id.node_begin = ih.node_begin;
+    id.node_end = ih.node_end;
+    id.ckey_begin = ih.ckey_begin;
+    id.ckey_end = ih.ckey_end;
+    id.nsides = ih.nsides;
+    const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
+    id.skin_half2 = sh * sh;
+    if (pp.enable_torque) h->any_torque = true;
+    if (ih.contact) h->any_contact = true;
+  }
+  h->grain_node_begin[G] = (uint32_t)N;
+  h->CK = (uint32_t)CK;
+
+  // ---- adjacency rows in the reference's NbdArr order
+  for (uint64_t i = 0; i < N; ++i) row_ptr[i + 1] = row_ptr[i] + h->deg[i];
+  slots = row_ptr[N];
+  h->slot0.assign(row_ptr.begin(), row_ptr.end());
+  std::vector<uint32_t> rows(slots);
+  {
+    std::vector<uint64_t> cur(row_ptr.begin(), row_ptr.end() - 1);
+    for (int s = 0; s < n; ++s) {
+      const pg_setup &st = setups[s];
+      const InstHost &ih = h->inst[s];
+      for (int g = 0; g < st.n_grains; ++g) {
+        const uint32_t base = ih.node_begin + (uint32_t)st.node_offset[g];
+        for (int64_t r = st.conn_offset[g]; r < st.conn_offset[g + 1]; ++r) {
+          const uint32_t ci = base + st.conn[2 * r], cj = base + st.conn[2 * r + 1];
+          rows[cur[ci]++] = cj;
+          rows[cur[cj]++] = ci;
+        }
+      }
+    }
+  }
+  {
+    std::vector<uint32_t> tmp;
+    for (uint64_t i = 0; i < N; ++i) {
+      tmp.assign(rows.begin() + row_ptr[i], rows.begin() + row_ptr[i + 1]);
+      std::sort(tmp.begin(), tmp.end());
+      if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end())
+        return fail(PG_ERR_INVALID, "Connectivity lists a bond twice (node " + std::to_string(i) + ")");
+    }
+  }
+  for (int s = 0; s < n; ++s) {
+    h->inst[s].slot_begin = row_ptr[h->inst[s].node_begin];
+    h->inst[s].slot_end = row_ptr[h->inst[s].node_end];
+  }
+  // ---- sliced ELL
+  const uint32_t S = (uint32_t)((N + SLICE - 1) / SLICE);
+  h->S = S;
+  h->slice_base.assign(S, 0);
+  h->slice_width.assign(S, 0);
+  uint64_t E = 0;
+  for (uint32_t sl = 0; sl < S; ++sl) {
+    uint32_t w = 0;
+    for (uint64_t i = (uint64_t)sl * SLICE; i < std::min<uint64_t>(N, (uint64_t)(sl + 1) * SLICE); ++i)
+      w = std::max(w, h->deg[i]);
+    h->slice_base[sl] = (uint32_t)E;
+    h->slice_width[sl] = w;
+    E += (uint64_t)w * SLICE;
+    if (E >= 0xFFFFFFFFull) return fail(PG_ERR_INVALID, "bond table exceeds 2^32 entries");
+  }
+  h->E = E;
+  h->bcol.assign(E, 0);
+  std::vector<uint8_t> bstat(E, BOND_PAD);
+  for (uint64_t i = 0; i < N; ++i) {
+    const uint32_t sl = (uint32_t)(i / SLICE), lane = (uint32_t)(i % SLICE);
+    for (uint32_t k = 0; k < h->deg[i]; ++k) {
+      const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
+      h->bcol[e] = rows[row_ptr[i] + k];
+      bstat[e] = BOND_INTACT;
+    }
+  }
+
+  // ---- device allocations
+  const size_t ND = (size_t)N * D;
+  RC(dalloc(h, h->rec[0], N));
+  RC(dalloc(h, h->rec[1], N));
+  RC(dalloc(h, h->vel[0], ND));
+  RC(dalloc(h, h->vel[1], ND));
+  RC(dalloc(h, h->acc, ND));
+  RC(dalloc(h, h->ext, ND));
+  RC(dalloc(h, h->xb, ND));
+  RC(dalloc(h, h->force_out, ND));
+  RC(dalloc(h, h->node_grain, N));
+  RC(dalloc(h, h->node_info, N));
+  RC(dalloc(h, h->ckey, N));
+  RC(dalloc(h, h->grains, G));
+  RC(dalloc(h, h->insts, n));
+  RC(dalloc(h, h->d_slice_base, S));
+  RC(dalloc(h, h->d_slice_width, S));
+  RC(dalloc(h, h->d_bcol, E));
+  RC(dalloc(h, h->bstat, E));
+  RC(dalloc(h, h->cl_ptr, (size_t)N + 1));
+  RC(dalloc(h, h->cl_cnt, (size_t)N + 1));
+  h->cl_cap = 1024;
+  RC(dalloc(h, h->cl_j, h->cl_cap));
+  RC(dalloc(h, h->cl_slot, h->cl_cap));
+  for (int k = 0; k < 3; ++k) RC(dalloc(h, h->bbox[k], (size_t)G * 2 * D));
+  RC(dalloc(h, h->wallpos[0], (size_t)n * 6));
+  RC(dalloc(h, h->wallpos[1], (size_t)n * 6));
+  RC(dalloc(h, h->gmean, h->any_torque ? (size_t)G * 3 : 1));
+  RC(dalloc(h, h->violation, 1));
+  RC(dalloc(h, h->wterm, (size_t)(CK ? CK : 1) * 24));
+  RC(dalloc(h, h->wflag, CK ? CK : 1));
+  RC(dalloc(h, h->reaction, (size_t)n * 18));
+  RC(dalloc(h, h->grids, n));
+  RC(dalloc(h, h->ibox, (size_t)n * 2 * D));
+  RC(dalloc(h, h->cell_of, N));
+  RC(dalloc(h, h->cell_nodes, N));
+  {
+    void *p = nullptr;
+    HIPCHK(hipHostMalloc(&p, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    h->h_pin = static_cast<uint32_t *>(p);
+  }
+
+  // ---- uploads
+  RC(h2d(h, h->rec[0], rec.data(), N));
+  RC(h2d(h, h->rec[1], rec.data(), N));
+  RC(h2d(h, h->vel[0], vel.data(), ND));
+  RC(h2d(h, h->vel[1], vel.data(), ND));
+  RC(h2d(h, h->acc, acc.data(), ND));
+  RC(h2d(h, h->ext, ext.data(), ND));
+  RC(h2d(h, h->node_grain, node_grain.data(), N));
+  RC(h2d(h, h->node_info, node_info.data(), N));
+  RC(h2d(h, h->ckey, ckey.data(), N));
+  RC(h2d(h, h->grains, grains.data(), G));
+  RC(h2d(h, h->insts, h->inst_dev.data(), (size_t)n));
+  RC(h2d(h, h->d_slice_base, h->slice_base.data(), S));
+  RC(h2d(h, h->d_slice_width, h->slice_width.data(), S));
+  RC(h2d(h, h->d_bcol, h->bcol.data(), E));
+  RC(h2d(h, h->bstat, bstat.data(), E));
+  std::vector<double> walls((size_t)n * 6);
+  for (int s = 0; s < n; ++s)
+    for (int k = 0; k < 6; ++k) walls[(size_t)s * 6 + k] = setups[s].wall[k];
+  RC(h2d(h, h->wallpos[0], walls.data(), walls.size()));
+  RC(h2d(h, h->wallpos[1], walls.data(), walls.size()));
+  HIPCHK(hipMemsetAsync(h->force_out, 0, ND * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(h->xb, 0, ND * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(h->reaction, 0, (size_t)n * 18 * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(h->wflag, 0, CK ? CK : 1, h->stream));
+  HIPCHK(hipMemsetAsync(h->wterm, 0, (size_t)(CK ? CK : 1) * 24 * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(h->cl_ptr, 0, ((size_t)N + 1) * sizeof(uint32_t), h->stream));
+  HIPCHK(hipMemsetAsync(h->gmean, 0, (h->any_torque ? (size_t)G * 3 : 1) * sizeof(double), h->stream));
+  for (int k = 0; k < 3; ++k) HIPCHK(launch_fill_box(h->bbox[k], (size_t)G * 2 * D, D, h->stream));
+  HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
+  fill_args(h);
+
+  // ---- position update of step 1 and the first neighbour list
+  HIPCHK(launch_prologue(D, h->A, h->stream));
+  RC(build_list(h, 1));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->t = 0;
+  return PG_OK;
+}
+
+void destroy_impl(pg_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
+  if (h->h_pin) (void)hipHostFree(h->h_pin);
+  for (int k = 0; k < 8; ++k)
+    if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int run_steps(pg_handle *h, uint32_t nsteps) {
+  if (nsteps == 0) return PG_OK;
+  const uint32_t t_end = h->t + nsteps;
+  if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
+  while (h->t < t_end) {
+    const uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
+    for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
+      if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
+      HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const uint32_t v = h->h_pin[0];
+    if (v <= h->t + chunk) {
+      if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid before the batch started");
+      h->skipped += (uint64_t)(h->t + chunk) - v + 1;
+      h->t = v - 1;
+      RC(build_list(h, v));
+      continue;
+    }
+    h->t += chunk;
+  }
+  HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag, h->reaction, h->stream));
+  return PG_OK;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char *pg_last_error(void) { return g_err.c_str(); }
+
+int pg_version(char *buf, size_t cap) {
+  const char *v = "libpg 0.1 (gfx950, FP64)";
+  if (!buf || cap == 0) return fail(PG_ERR_INVALID, "null buffer");
+  std::snprintf(buf, cap, "%s", v);
+  return PG_OK;
+}
+
+int pg_batch_create(const pg_setup *setups, const pg_params *params, int n, int device,
+                    pg_handle **out) {
+  if (!out || !setups || !params) return fail(PG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  pg_handle *h = nullptr;
+  try {
+    h = new pg_handle();
+    int rc = create_impl(setups, params, n, device, h);
+    if (rc != PG_OK) {
+      std::string keep = g_err;
+      destroy_impl(h);
+      g_err = keep;
+      return rc;
+    }
+  } catch (const std::bad_alloc &) {
+    destroy_impl(h);
+    return fail(PG_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception &e) {
+    destroy_impl(h);
+    return fail(PG_ERR_INVALID, e.what());
+  }
+  *out = h;
+  return PG_OK;
+}
+
+int pg_create(const pg_setup *setup, const pg_params *params, int device, pg_handle **out) {
+  return pg_batch_create(setup, params, 1, device, out);
+}
+
+int pg_destroy(pg_handle *h) {
+  destroy_impl(h);
+  return PG_OK;
+}
+
+int pg_step(pg_handle *h, uint32_t nsteps) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  return run_steps(h, nsteps);
+}
+
+int pg_sync(pg_handle *h) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+int pg_current_step(pg_handle *h, uint32_t *t) {
+  if (!h || !t) return fail(PG_ERR_INVALID, "null argument");
+  *t = h->t;
+  return PG_OK;
+}
+
+int pg_sizes(pg_handle *h, int inst, uint64_t *n_nodes, uint64_t *n_grains, uint64_t *n_slots) {
+  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad handle or instance");
+  const InstHost &ih = h->inst[inst];
+  if (n_nodes) *n_nodes = ih.node_end - ih.node_begin;
+  if (n_grains) *n_grains = ih.grain_end - ih.grain_begin;
+  if (n_slots) *n_slots = ih.slot_end - ih.slot_begin;
+  return PG_OK;
+}
+
+int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
+  if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  const int D = h->dim;
+  const InstHost &ih = h->inst[inst];
+  const size_t n = ih.node_end - ih.node_begin;
+  HIPCHK(hipSetDevice(h->device));
+  switch (field) {
+    case PG_FIELD_CURRPOS:
+    case PG_FIELD_DISP: {
+      std::vector<NodeRec> r(n);
+      RC(d2h(h, r.data(), h->rec[h->t & 1] + ih.node_begin, n));
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < D; ++k)
+          dst[i * D + k] = field == PG_FIELD_DISP ? r[i].u[k] : r[i].X[k] + r[i].u[k];
+      return PG_OK;
+    }
+    case PG_FIELD_VEL:
+      return d2h(h, dst, h->vel[h->t & 1] + (size_t)ih.node_begin * D, n * D);
+    case PG_FIELD_ACC:
+      return d2h(h, dst, h->acc + (size_t)ih.node_begin * D, n * D);
+    case PG_FIELD_FORCE:
+      return d2h(h, dst, h->force_out + (size_t)ih.node_begin * D, n * D);
+    default:
+      return fail(PG_ERR_INVALID, "unknown field");
+  }
+}
+
+int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
+  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  const int D = h->dim;
+  if (wall_info) RC(d2h(h, wall_info, h->wallpos[(h->t + 1) & 1] + (size_t)inst * 6, 6));
+  if (reaction) {
+    double r[18];
+    RC(d2h(h, r, h->reaction + (size_t)inst * 18, 18));
+    for (int s = 0; s < PG_MAX_SIDES; ++s)
+      for (int k = 0; k < D; ++k) reaction[s * D + k] = r[s * 3 + k];
+  }
+  return PG_OK;
+}
+
+int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst) {
+  if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  std::vector<uint8_t> st(h->E);
+  RC(d2h(h, st.data(), h->bstat, h->E));
+  const InstHost &ih = h->inst[inst];
+  for (uint32_t i = ih.node_begin; i < ih.node_end; ++i) {
+    const uint32_t sl = i / SLICE, lane = i % SLICE;
+    for (uint32_t k = 0; k < h->deg[i]; ++k)
+      dst[h->slot0[i] - ih.slot_begin + k] =
+          st[h->slice_base[sl] + (uint64_t)SLICE * k + lane] == BOND_INTACT ? 1 : 0;
+  }
+  return PG_OK;
+}
+
+int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_rows,
+                 uint64_t *rows) {
+  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  const InstHost &ih = h->inst[inst];
+  if (grain < 0 || grain >= (int)(ih.grain_end - ih.grain_begin))
+    return fail(PG_ERR_INVALID, "bad grain");
+  const uint32_t gg = ih.grain_begin + grain;
+  const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
+  // status of this grain's rows only
+  std::vector<uint8_t> st(h->E);
+  RC(d2h(h, st.data(), h->bstat, h->E));
+  uint64_t r = 0;
+  for (uint32_t i = a; i < b; ++i) {
+    const uint32_t sl = i / SLICE, lane = i % SLICE;
+    for (uint32_t k = 0; k < h->deg[i]; ++k) {
+      const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
+      if (st[e] != BOND_INTACT) continue;
+      const uint32_t q = h->bcol[e] - a, li = i - a;
+      if (li < q) {
+        if (dst && r < cap_rows) {
+          dst[2 * r] = li;
+          dst[2 * r + 1] = q;
+        }
+        ++r;
+      }
+    }
+  }
+  if (rows) *rows = r;
+  return PG_OK;
+}
+
+int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs, uint64_t *pairs) {
+  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  if (h->t == 0) return fail(PG_ERR_STATE, "no step has been taken");
+  const int D = h->dim;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemsetAsync(h->cl_cnt, 0, ((size_t)h->N + 1) * sizeof(uint32_t), h->stream));
+  HIPCHK(launch_pairs(D, false, h->A, h->t, h->cl_cnt, nullptr, nullptr, h->stream));
+  uint32_t *ptr = nullptr;
+  RC(dalloc(h, ptr, (size_t)h->N + 1));
+  int rc = scan(h, h->cl_cnt, ptr, (size_t)h->N + 1);
+  if (rc) {
+    dfree(h, ptr);
+    return rc;
+  }
+  uint32_t total = 0;
+  rc = d2h(h, &total, ptr + h->N, 1);
+  uint32_t *out = nullptr;
+  if (!rc) rc = dalloc(h, out, (size_t)2 * (total ? total : 1));
+  if (!rc) {
+    hipError_t e = launch_pairs(D, true, h->A, h->t, nullptr, ptr, out, h->stream);
+    if (e != hipSuccess) rc = fail(PG_ERR_DEVICE, hipGetErrorString(e));
+  }
+  std::vector<uint32_t> all((size_t)2 * total);
+  if (!rc) rc = d2h(h, all.data(), out, all.size());
+  dfree(h, ptr);
+  dfree(h, out);
+  if (rc) return rc;
+  const InstHost &ih = h->inst[inst];
+  uint64_t np = 0;
+  for (uint64_t q = 0; q < total; ++q) {
+    const uint32_t i = all[2 * q], j = all[2 * q + 1];
+    if (i < ih.node_begin || i >= ih.node_end) continue;
+    if (dst && np < cap_pairs) {
+      dst[2 * np] = i - ih.node_begin;
+      dst[2 * np + 1] = j - ih.node_begin;
+    }
+    ++np;
+  }
+  if (pairs) *pairs = np;
+  return PG_OK;
+}
+
+int pg_stream(pg_handle *h, void **stream) {
+  if (!h || !stream) return fail(PG_ERR_INVALID, "null argument");
+  *stream = (void *)h->stream;
+  return PG_OK;
+}
+
+int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries, uint64_t *skipped_steps) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  if (rebuilds) *rebuilds = h->rebuilds;
+  if (list_entries) *list_entries = h->list_entries;
+  if (skipped_steps) *skipped_steps = h->skipped;
+  return PG_OK;
+}
+
+int pg_set_tuning(pg_handle *h, double skin_factor, int block) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  if (!(skin_factor > 0) || skin_factor > 16)
+    return fail(PG_ERR_INVALID, "skin_factor must be in (0, 16]");
+  if (block != 64 && block != 128 && block != 256)
+    return fail(PG_ERR_INVALID, "block must be 64, 128 or 256");
+  HIPCHK(hipSetDevice(h->device));
+  h->block = block;
+  h->skin_factor = skin_factor;
+  for (uint32_t s = 0; s < h->I; ++s) {
+    InstHost &ih = h->inst[s];
+    ih.skin = skin_factor * (ih.Rc > 0 ? ih.Rc : 0.0);
+    const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
+    h->inst_dev[s].skin_half2 = sh * sh;
+  }
+  RC(h2d(h, h->insts, h->inst_dev.data(), h->I));
+  // the next step to run is t+1: rebuild from its positions
+  return build_list(h, h->t + 1);
+}
+
+int pg_set_graph(pg_handle *h, int use_graph) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  (void)use_graph;
+  return PG_OK;
+}
+
+// Events on the handle's stream, for timing by the caller.
+int pg_event_record(pg_handle *h, int slot) {
+  if (!h || slot < 0 || slot >= 8) return fail(PG_ERR_INVALID, "bad event slot");
+  HIPCHK(hipEventRecord(h->ev[slot], h->stream));
+  return PG_OK;
+}
+
+int pg_event_elapsed(pg_handle *h, int a, int b, float *ms) {
+  if (!h || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(PG_ERR_INVALID, "bad event slot");
+  HIPCHK(hipEventSynchronize(h->ev[b]));
+  HIPCHK(hipEventElapsedTime(ms, h->ev[a], h->ev[b]));
+  return PG_OK;
+}
+
+} // extern "C"
+
+extern "C" int pg_device_count(int *n) {
+  if (!n) return fail(PG_ERR_INVALID, "null argument");
+  *n = 0;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e == hipErrorNoDevice) return PG_OK;
+  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  *n = c;
+  return PG_OK;
+}
+
+extern "C" int pg_check_arith(int device, uint64_t n, const double *a, const double *b, double *q,
+                              double *r) {
+  if (!a || !b || !q || !r) return fail(PG_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(device));
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, 4 * (n ? n : 1) * sizeof(double)));
+  const size_t bytes = n * sizeof(double);
+  hipError_t e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_arith(n, d, d + n, d + 2 * n, d + 3 * n, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(q, d + 2 * n, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(r, d + 3 * n, bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("pg_check_arith: ") + hipGetErrorString(e));
+  return PG_OK;
+}

@@ -1,0 +1,42 @@
+// pg_kernels.h -- host-side launchers of the gfx950 kernels in pg_kernels.hip.
+#ifndef PG_KERNELS_H
+#define PG_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pg_internal.h"
+
+namespace pg {
+
+hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int block,
+                       hipStream_t s);
+hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s);
+hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gmean,
+                             hipStream_t s);
+hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
+                           const double *wterm, const uint8_t *wflag, double *reaction,
+                           hipStream_t s);
+hipError_t launch_fill_box(unsigned long long *p, size_t n, int dim, hipStream_t s);
+hipError_t launch_inst_box(int dim, const StepArgs &A, uint32_t t, unsigned long long *ibox,
+                           hipStream_t s);
+hipError_t launch_cell_count(int dim, const StepArgs &A, uint32_t t, const GridDev *grids,
+                             uint32_t *cell_of, uint32_t *cell_cnt, hipStream_t s);
+hipError_t launch_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t *cell_start,
+                            uint32_t *cell_cur, uint32_t *cell_nodes, hipStream_t s);
+hipError_t launch_list(int dim, bool fill, const StepArgs &A, uint32_t t, const GridDev *grids,
+                       const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb,
+                       hipStream_t s);
+hipError_t launch_pairs(int dim, bool fill, const StepArgs &A, uint32_t t, uint32_t *cnt,
+                        const uint32_t *ptr, uint32_t *out, hipStream_t s);
+hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
+                        hipStream_t s);
+// hipcub exclusive sum; call with tmp=nullptr to size *tmp_bytes.
+hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp,
+                              size_t *tmp_bytes, hipStream_t s);
+
+} // namespace pg
+
+#endif

@@ -1,0 +1,1012 @@
+// pg_kernels.hip -- gfx950 kernels of libpg (HIP, FP64).
+//
+// Arithmetic contract: every floating-point expression below is written in
+// the operation order of the reference (file:line cited per function) and
+// the file is compiled with -ffp-contract=off, so each +,-,*,/ is a single
+// IEEE binary64 rounding and no FMA is formed.  sqrt and / are the
+// correctly rounded HIP forms.  3-component norms and dot products sum as
+// (a0*b0 + a1*b1) + a2*b2, the order of Eigen 3.3.9's vectorised redux that
+// the reference's norm()/dot() use.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "pg_internal.h"
+#include "pg_kernels.h"
+
+namespace pg {
+
+// ---------------------------------------------------------------- helpers
+
+template <int D> __device__ __forceinline__ double red(const double *a, const double *b);
+template <> __device__ __forceinline__ double red<2>(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1];
+}
+template <> __device__ __forceinline__ double red<3>(const double *a, const double *b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// Logical block index that gives each XCD (blocks are dealt round-robin to
+// the 8 XCDs) one contiguous range of nodes, so neighbour gathers of a block
+// hit the L2 of the XCD that already holds its neighbours.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7;
+  const uint32_t x = b & 7, idx = b >> 3;
+  return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+__device__ __forceinline__ bool grain_movable(const GrainDev &gp, const InstDev &ip,
+                                              uint32_t g, uint32_t t) {
+  return (gp.flags & GF_MOVABLE) ||
+         (ip.set_movable_grain == (int32_t)g && t >= ip.set_movable_step);
+}
+__device__ __forceinline__ bool grain_stoppable(const GrainDev &gp, const InstDev &ip,
+                                                uint32_t g, uint32_t t) {
+  return (gp.flags & GF_STOPPABLE) ||
+         (ip.set_stoppable_grain == (int32_t)g && t >= ip.set_stoppable_step);
+}
+
+// Current reference position (pos, including a reset_partzero_y shift).
+template <int D>
+__device__ __forceinline__ void cur_pos(const NodeRec &r, const GrainDev &gp, double *X) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) X[k] = r.X[k];
+  if (gp.flags & GF_YSHIFT) X[1] = X[1] + gp.yshift;
+}
+
+// bbox key arrays are [grain][2*D]: mins then maxes.
+template <int D>
+__device__ __forceinline__ void load_box(const unsigned long long *bb, uint32_t g, double *b) {
+#pragma unroll
+  for (int k = 0; k < 2 * D; ++k) b[k] = dkey_inv(bb[(size_t)g * 2 * D + k]);
+}
+
+// intersects_box_fext (contact.h:983-1020).  Boxes are stored as
+// [min_0..min_{D-1}, max_0..max_{D-1}]; in 2D the reference's layout is
+// [xmin,xmax,ymax,ymin], so its A_Left/A_Right/A_Top/A_Bottom are
+// A[0]/A[2]/A[3]/A[1] here.  In 3D the two layouts coincide.
+template <int D>
+__device__ __forceinline__ bool boxes_meet(const double *A, const double *B, double c) {
+  if (D == 2) {
+    const double AL = A[0] - c, AR = A[2] + c, AT = A[3] + c, AB = A[1] - c;
+    return (AL <= B[2]) && (AR >= B[0]) && (AT >= B[1]) && (AB <= B[3]);
+  } else {
+    const double ax0 = A[0] - c, ay0 = A[1] - c, az0 = A[2] - c;
+    const double ax1 = A[3] + c, ay1 = A[4] + c, az1 = A[5] + c;
+    return (ax0 <= B[3]) && (ax1 >= B[0]) && (ay1 >= B[1]) && (ay0 <= B[4]) &&
+           (az0 <= B[5]) && (az1 >= B[2]);
+  }
+}
+
+// within_interior_fext (contact.h:1066-1080) on a [min..,max..] box `bx`
+// and the walls in lrtb() order.  The reference indexes the first four
+// entries of get_minmax's output whatever the dimension: 2D
+// [xmin,xmax,ymax,ymin] vs [left,right,top,bottom]; 3D [xmin,ymin,zmin,xmax]
+// vs [x_min,y_min,z_min,x_max] (so in 3D the test is practically never
+// true and the walls are always evaluated).
+template <int D>
+__device__ __forceinline__ bool within_interior(const double *bx, const double *w, double c) {
+  double A0, A1, A2, A3;
+  if (D == 2) {
+    A0 = bx[0]; A1 = bx[2]; A2 = bx[3]; A3 = bx[1];
+  } else {
+    A0 = bx[0]; A1 = bx[1]; A2 = bx[2]; A3 = bx[3];
+  }
+  const double AL = A0 - c, AR = A1 + c, AT = A2 + c, AB = A3 - c;
+  return (w[0] < AL) && (AR < w[1]) && (w[2] > AT) && (w[3] < AB);
+}
+
+// segment_mass<dim> (contact.h:16-41), returns area, writes centroid.
+template <int D>
+__device__ __forceinline__ double segment_mass(double R, double d, double &centroid) {
+  double Area, I;
+  if (D == 2) {
+    Area = M_PI * (R * R) / 2 - (R * R) * asin(d / R) - sqrt(R * R - d * d) * d;
+    I = 0.0; // 2 / 3 * pow(..., 1.5) with integer 2/3 == 0 and a finite pow
+  } else {
+    Area = M_PI / 3 * ((R - d) * (R - d)) * (2 * R - d);
+    I = M_PI / 4 * ((R * R - d * d) * (R * R - d * d));
+  }
+  centroid = I / Area;
+  return Area;
+}
+
+// wall_side_force_update (contact.h:310-431).  Returns true if the node is
+// within contact distance of this side; t4 receives the reaction terms
+// {normal, damping, friction t1, friction t2} (each already vol-scaled).
+template <int D>
+__device__ __forceinline__ bool wall_side(double dist, int dir, const InstDev &ip,
+                                          double rho, double vol, double wspeed,
+                                          const double *v, double *F, double *t4) {
+  if (!(fabs(dist) <= ip.Rc)) return false;
+  double cen;
+  const double area = segment_mass<D>(ip.Rc, fabs(dist), cen);
+  const double contrib = (ip.Rc - cen);
+  const double pos = (contrib > 0 ? contrib : 0);
+  const double Fw = (ip.Kn * pos * area);
+  const double sgn = (dist > 0 ? 1.0 : -1.0);
+  F[dir] += (sgn * Fw);
+  t4[0] = (vol * sgn * Fw);
+  t4[1] = 0.0;
+  t4[2] = 0.0;
+  t4[3] = 0.0;
+  double rel[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) rel[k] = v[k];
+  rel[dir] -= wspeed;
+  if (ip.flags & IF_FRICTION) {
+    if (D == 2) {
+      const int tg = (dir + 1) % 2;
+      if (fabs(rel[tg]) > 0) {
+        const double sg = (rel[tg] > 0 ? 1.0 : -1.0);
+        double ff = (-ip.mu) * fabs(Fw) * sg;
+        if (fabs(rel[tg]) < ip.mu * fabs(Fw) * ip.dt / rho) ff = -rel[tg] * rho / ip.dt;
+        F[tg] += ff;
+        t4[2] = (vol * ff);
+      }
+    } else {
+      const int t1 = (dir + 1) % 3, t2 = (dir + 2) % 3;
+      const double vp1 = rel[t1], vp2 = rel[t2];
+      const double vpn = sqrt(vp1 * vp1 + vp2 * vp2);
+      if (vpn > 0) {
+        const double u1 = vp1 / vpn, u2 = vp2 / vpn;
+        double ff1 = (-ip.mu) * fabs(Fw) * u1;
+        double ff2 = (-ip.mu) * fabs(Fw) * u2;
+        if (vpn < ip.mu * fabs(Fw) * ip.dt / rho) {
+          ff1 = -vpn / rho / ip.dt * u1;
+          ff2 = -vpn / rho / ip.dt * u2;
+        }
+        F[t1] += ff1;
+        F[t2] += ff2;
+        t4[2] = (vol * ff1);
+        t4[3] = (vol * ff2);
+      }
+    }
+  }
+  if (ip.flags & IF_DAMPING) {
+    const double cd = 2 * ip.zeta * sqrt(ip.Kn * rho) * sqrt(fabs(area));
+    const double fd = (-cd) * rel[dir];
+    F[dir] += fd;
+    t4[1] = (vol * fd);
+  }
+  return true;
+}
+
+// Normal, damping and friction force of one contact pair
+// (contact.h:735-808).  d = x_j - x_i, r = |d|, R/K the pair's radius and
+// stiffness, vj the partner's volume, w = v_i - v_j.
+template <int D>
+__device__ __forceinline__ void pair_force(const double *d, double r, double R, double K,
+                                           double vj, const double *w, double rho,
+                                           const InstDev &ip, double *F) {
+  double e[D], fc[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) e[k] = d[k] / r;
+  const double g = -K * (R - r) * vj;
+#pragma unroll
+  for (int k = 0; k < D; ++k) fc[k] = g * e[k];
+  const double fcn = sqrt(red<D>(fc, fc));
+#pragma unroll
+  for (int k = 0; k < D; ++k) F[k] += fc[k];
+  if (ip.flags & IF_DAMPING) {
+    const double p = red<D>(w, e);
+    const double cd = 2 * ip.zeta * sqrt(K * rho) * sqrt(vj);
+#pragma unroll
+    for (int k = 0; k < D; ++k) F[k] += (-cd) * (p * e[k]);
+  }
+  if (ip.flags & IF_FRICTION) {
+    const double p = red<D>(w, e);
+    double tt[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) tt[k] = w[k] - p * e[k];
+    const double tau = sqrt(red<D>(tt, tt));
+    if (tau > 0) {
+      double ut[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) ut[k] = tt[k] / tau;
+      double s = -ip.mu * fcn;
+      if (tau < ip.mu * fcn * ip.dt / rho) s = -tau / rho / ip.dt;
+#pragma unroll
+      for (int k = 0; k < D; ++k) F[k] += s * ut[k];
+    }
+  }
+}
+
+// Contact partner test shared by the step kernel and the pair readback:
+// returns true and fills (d, r, R, K) if partner list entry c of node i is
+// in contact at step t (the gates of timeloop.h:698-747 and the self-contact
+// rules of contact.h:686-733).
+template <int D>
+__device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, uint32_t c,
+                                               uint32_t g, const GrainDev &gp,
+                                               const InstDev &ip, const double *xc,
+                                               const double *Xc, bool selfc, bool stop,
+                                               const double *boxg, const NodeRec *Rr,
+                                               uint32_t &j, double *d, double &r,
+                                               double &R, double &K) {
+  j = A.cl_j[c];
+  const uint32_t gj = A.node_grain[j];
+  R = ip.Rc;
+  K = ip.Kn;
+  const NodeRec rj = Rr[j];
+  double Xj[D];
+  if (gj == g) {
+    if (!selfc) return false;
+    const int32_t bs = A.cl_slot[c];
+    if (bs >= 0 && A.bstat[bs] == BOND_INTACT) return false;
+    cur_pos<D>(rj, gp, Xj);
+    double dX[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dX[k] = Xc[k] - Xj[k];
+    const double rd = sqrt(red<D>(dX, dX));
+    if (rd < ip.Rc) {
+      R = rd;
+      K = gp.cnot / rd;
+    } else {
+      R = ip.Rc;
+      K = gp.cnot / ip.Rc;
+    }
+  } else {
+    if (!stop) return false;
+    double boxj[2 * D];
+    load_box<D>(A.bbox[t % 3], gj, boxj);
+    if (!boxes_meet<D>(boxg, boxj, ip.Rc)) return false;
+    const GrainDev gpj = A.grains[gj];
+    cur_pos<D>(rj, gpj, Xj);
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) d[k] = (Xj[k] + rj.u[k]) - xc[k];
+  r = sqrt(red<D>(d, d));
+  return r <= R;
+}
+
+// Wavefront min/max of D coordinates grouped by `grp` (uniform group fast
+// path), combined into keys[grp][2*D] with 64-bit atomics.
+template <int D>
+__device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32_t grp,
+                                                bool valid, const double *x) {
+  unsigned long long kmin[D], kmax[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    kmin[k] = valid ? dkey(x[k]) : ~0ull;
+    kmax[k] = valid ? dkey(x[k]) : 0ull;
+  }
+  const uint32_t g0 = __builtin_amdgcn_readfirstlane(grp);
+  const bool uniform = __all((!valid) || grp == g0);
+  if (uniform) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const unsigned long long a = __shfl_xor(kmin[k], off, 64);
+        const unsigned long long b = __shfl_xor(kmax[k], off, 64);
+        kmin[k] = a < kmin[k] ? a : kmin[k];
+        kmax[k] = b > kmax[k] ? b : kmax[k];
+      }
+    }
+    if ((threadIdx.x & 63) == 0 && g0 != 0xFFFFFFFFu) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        atomicMin(&keys[(size_t)g0 * 2 * D + k], kmin[k]);
+        atomicMax(&keys[(size_t)g0 * 2 * D + D + k], kmax[k]);
+      }
+    }
+  } else if (valid) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      atomicMin(&keys[(size_t)grp * 2 * D + k], kmin[k]);
+      atomicMax(&keys[(size_t)grp * 2 * D + D + k], kmax[k]);
+    }
+  }
+}
+
+// ------------------------------------------------------------ step kernel
+
+// One full time step t for every node (timeloop.h:608-878):
+//   PMB bond force + bond breaking        particle2.h:112-152, 195-230
+//   self contact, inter-grain contact     timeloop.h:709-747, contact.h:667-812
+//   wall contact                          timeloop.h:753-769, contact.h:899-964
+//   external force, torque                timeloop.h:774-824
+//   velocity Verlet + clamped nodes       timeloop.h:834-863
+//   wall motion                           timeloop.h:866-878
+// followed by the position update of step t+1 (timeloop.h:661-669), the
+// grain bounding boxes of step t+1 (get_minmax, contact.h:1023-1062) and
+// the Verlet-skin check.  Reads u^t from rec[t&1] and v^{t-1} from
+// vel[(t+1)&1]; writes u^{t+1} to rec[(t+1)&1] and v^t to vel[t&1].
+template <int D, bool RECORD>
+__global__ __launch_bounds__(256) void k_step(const StepArgs A, const uint32_t t) {
+  if (*A.violation <= t) return; // neighbour list stale since an earlier step
+  const uint32_t nb = gridDim.x;
+  const uint32_t lb = xcd_block(blockIdx.x, nb);
+  const uint32_t i = lb * blockDim.x + threadIdx.x;
+  const uint32_t nthreads = nb * blockDim.x;
+
+  { // housekeeping: clear the box buffer of step t+2, move the walls
+    unsigned long long *bz = A.bbox[(t + 2) % 3];
+    const uint32_t nz = A.n_grains * 2 * D;
+    for (uint32_t q = i; q < nz; q += nthreads) bz[q] = ((q % (2 * D)) < D) ? ~0ull : 0ull;
+    if (i < A.n_insts * 6) {
+      const InstDev &ip = A.insts[i / 6];
+      A.wallpos[(t + 1) & 1][i] = A.wallpos[t & 1][i] + ip.speed[i % 6] * ip.dt;
+    }
+  }
+
+  const bool valid = i < A.n_nodes;
+  uint32_t g = 0xFFFFFFFFu;
+  double xn[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xn[k] = 0.0;
+  bool viol = false;
+
+  if (valid) {
+    g = A.node_grain[i];
+    const GrainDev gp = A.grains[g];
+    const InstDev &ip = A.insts[gp.inst];
+    const NodeRec *Rr = A.rec[t & 1];
+    const NodeRec ri = Rr[i];
+    double Xo[D], Xc[D], ui[D], xc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      Xo[k] = ri.X[k];
+      ui[k] = ri.u[k];
+    }
+    cur_pos<D>(ri, gp, Xc);
+#pragma unroll
+    for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
+    const uint8_t info = A.node_info[i];
+    const bool mov = grain_movable(gp, ip, g, t);
+    const double *vprev = A.vel[(t + 1) & 1];
+    double *vcur = A.vel[t & 1];
+    double F[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) F[k] = 0.0;
+    double vi[D], ai[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      vi[k] = vprev[(size_t)i * D + k];
+      ai[k] = A.acc[(size_t)i * D + k];
+    }
+    const uint32_t key = A.ckey[i];
+    bool wall_hit = false;
+    double wt[6][4];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) wt[s][q] = 0.0;
+
+    if (mov) {
+      // ---- (a) PMB bond force with bond breaking
+      const bool brk = (ip.flags & IF_FRACTURE) && (gp.flags & GF_BREAKABLE);
+      const uint32_t sl = i >> 6, lane = i & 63;
+      const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+      for (uint32_t k = 0; k < width; ++k) {
+        const uint32_t e = base + 64u * k + lane;
+        if (A.bstat[e] != BOND_INTACT) continue;
+        const uint32_t j = A.bcol[e];
+        const NodeRec rj = Rr[j];
+        double xi[D], eta[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) xi[q] = rj.X[q] - Xo[q];
+        const double xin = sqrt(red<D>(xi, xi));
+#pragma unroll
+        for (int q = 0; q < D; ++q) eta[q] = (rj.u[q] - ui[q]) + xi[q];
+        const double n = sqrt(red<D>(eta, eta));
+        const double s = (n - xin) / xin;
+        const double w = (gp.cnot * s) * rj.X[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) F[q] = F[q] + w * (eta[q] / n);
+        if (brk && s > gp.snot) A.bstat[e] = BOND_BROKEN;
+      }
+
+      // ---- (b) contact from the sorted Verlet list
+      const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
+      const bool stop = grain_stoppable(gp, ip, g, t);
+      double boxg[2 * D];
+      load_box<D>(A.bbox[t % 3], g, boxg);
+      if ((info & NI_CONTACT) && (selfc || stop)) {
+        const uint32_t c0 = A.cl_ptr[i], c1 = A.cl_ptr[i + 1];
+        for (uint32_t c = c0; c < c1; ++c) {
+          uint32_t j;
+          double d[D], r, R, K;
+          if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, boxg, Rr, j, d,
+                                  r, R, K))
+            continue;
+          double w[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) w[k] = vi[k] - vprev[(size_t)j * D + k];
+          pair_force<D>(d, r, R, K, Rr[j].X[D], w, gp.rho, ip, F);
+        }
+      }
+
+      // ---- (c) walls
+      if (stop && (ip.flags & IF_WALL) && (info & NI_CONTACT)) {
+        const double *wp = A.wallpos[t & 1] + (size_t)gp.inst * 6;
+        if (!within_interior<D>(boxg, wp, ip.Rc)) {
+          const double vol = ri.X[D];
+          if (D == 2) {
+            wall_hit |= wall_side<D>(xc[1] - wp[3], 1, ip, gp.rho, vol, ip.speed[3], vi, F, wt[3]);
+            wall_hit |= wall_side<D>(xc[0] - wp[0], 0, ip, gp.rho, vol, ip.speed[0], vi, F, wt[0]);
+            wall_hit |= wall_side<D>(xc[0] - wp[1], 0, ip, gp.rho, vol, ip.speed[1], vi, F, wt[1]);
+            wall_hit |= wall_side<D>(xc[1] - wp[2], 1, ip, gp.rho, vol, ip.speed[2], vi, F, wt[2]);
+          } else {
+            wall_hit |= wall_side<D>(xc[0] - wp[0], 0, ip, gp.rho, vol, ip.speed[0], vi, F, wt[0]);
+            wall_hit |= wall_side<D>(xc[0] - wp[3], 0, ip, gp.rho, vol, ip.speed[3], vi, F, wt[3]);
+            wall_hit |= wall_side<D>(xc[1] - wp[1], 1, ip, gp.rho, vol, ip.speed[1], vi, F, wt[1]);
+            wall_hit |= wall_side<D>(xc[1] - wp[4], 1, ip, gp.rho, vol, ip.speed[4], vi, F, wt[4]);
+            wall_hit |= wall_side<D>(xc[2] - wp[2], 2, ip, gp.rho, vol, ip.speed[2], vi, F, wt[2]);
+            wall_hit |= wall_side<D>(xc[2] - wp[5], 2, ip, gp.rho, vol, ip.speed[5], vi, F, wt[5]);
+          }
+        }
+      }
+
+      // ---- external force (timeloop.h:776-786) and torque (:790-824)
+      if (ip.flags & IF_GRAD_EXT) {
+        double sc = (double)t / (double)ip.extforce_maxstep;
+        if (sc > 1) sc = 1;
+#pragma unroll
+        for (int k = 0; k < D; ++k) F[k] += sc * A.ext[(size_t)i * D + k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) F[k] += A.ext[(size_t)i * D + k];
+      }
+      if (ip.flags & IF_TORQUE) {
+        const double *cm = A.gmean + (size_t)g * 3;
+        const uint32_t ta = gp.torque_axis;
+        double rv[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) rv[k] = xc[k] - cm[k];
+        if (D == 3) rv[ta] = 0;
+        rp[(ta + 1) % 3] = -rv[(ta + 2) % 3];
+        rp[(ta + 2) % 3] = rv[(ta + 1) % 3];
+        const double pn = sqrt(red<D>(rp, rp));
+        if (pn > 0) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) rp[k] = rp[k] / pn;
+        } else {
+          rp[(ta + 1) % 3] = 0;
+          rp[(ta + 2) % 3] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) F[k] += (gp.torque_val * rp[k]);
+      }
+
+      // ---- (d) velocity Verlet with clamped nodes (timeloop.h:834-863)
+      double an[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        an[k] = (1 / gp.rho) * F[k];
+        double tmp = ai[k] + an[k];
+        tmp *= ip.half_dt;
+        vi[k] = vi[k] + tmp;
+        ai[k] = an[k];
+      }
+      if (info & NI_CLAMPED) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          ui[k] = 0.0;
+          vi[k] = 0.0;
+          ai[k] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) A.acc[(size_t)i * D + k] = ai[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) vcur[(size_t)i * D + k] = vi[k];
+
+    // ---- position update of step t+1 (timeloop.h:661-669)
+    double un[D];
+    if (grain_movable(gp, ip, g, t + 1)) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) un[k] = ui[k] + ((ip.dt * vi[k]) + (ip.h * ai[k]));
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) un[k] = ui[k];
+    }
+    NodeRec *Rn = A.rec[(t + 1) & 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      Rn[i].u[k] = un[k];
+      xn[k] = Xc[k] + un[k];
+    }
+    if (info & NI_CONTACT) {
+      double dd[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) dd[k] = xn[k] - A.xb[(size_t)i * D + k];
+      viol = red<D>(dd, dd) > ip.skin_half2;
+    }
+    if (RECORD) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) A.force_out[(size_t)i * D + k] = F[k];
+      if (key != 0xFFFFFFFFu) {
+        A.wflag[key] = wall_hit ? 1 : 0;
+        if (wall_hit) {
+          const int ns = (D == 2) ? 4 : 6;
+          for (int s = 0; s < ns; ++s)
+            for (int q = 0; q < 4; ++q) A.wterm[((size_t)key * 6 + s) * 4 + q] = wt[s][q];
+        }
+      }
+    }
+  }
+
+  if (viol) atomicMin(A.violation, t + 1);
+  wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
+}
+
+// Position update of step 1 from the initial state (timeloop.h:661-669),
+// plus the grain boxes of step 1.
+template <int D>
+__global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < A.n_nodes;
+  uint32_t g = 0xFFFFFFFFu;
+  double xn[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) xn[k] = 0.0;
+  if (valid) {
+    g = A.node_grain[i];
+    const GrainDev gp = A.grains[g];
+    const InstDev &ip = A.insts[gp.inst];
+    const NodeRec ri = A.rec[0][i];
+    double Xc[D], un[D];
+    cur_pos<D>(ri, gp, Xc);
+    const bool mov = grain_movable(gp, ip, g, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double v = A.vel[0][(size_t)i * D + k];
+      const double a = A.acc[(size_t)i * D + k];
+      un[k] = mov ? ri.u[k] + ((ip.dt * v) + (ip.h * a)) : ri.u[k];
+      A.rec[1][i].u[k] = un[k];
+      xn[k] = Xc[k] + un[k];
+    }
+  }
+  wave_box_atomic<D>(A.bbox[1], g, valid, xn);
+}
+
+// mean_CurrPos of every grain (particle2.h:89-98), sequential per grain.
+template <int D>
+__global__ void k_grain_mean(const StepArgs A, const uint32_t t, double *gmean) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.n_grains) return;
+  const GrainDev gp = A.grains[g];
+  const NodeRec *Rr = A.rec[t & 1];
+  double m[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) m[k] = 0.0;
+  for (uint32_t n = gp.node_begin; n < gp.node_end; ++n) {
+    double Xc[D];
+    cur_pos<D>(Rr[n], gp, Xc);
+#pragma unroll
+    for (int k = 0; k < D; ++k) m[k] += (Xc[k] + Rr[n].u[k]);
+  }
+  const uint32_t nn = gp.node_end - gp.node_begin;
+#pragma unroll
+  for (int k = 0; k < D; ++k) gmean[(size_t)g * 3 + k] = m[k] / (double)nn;
+}
+
+// Wall reaction of a recorded step, summed in the reference's sequential
+// order (grain, node in contact order, then side): one block per instance.
+// contact.h:339,374,411-412,428 via timeloop.h:753-769.
+template <int D>
+__global__ __launch_bounds__(256) void k_reaction(const InstDev *insts, const double *wterm,
+                                                  const uint8_t *wflag, double *reaction) {
+  constexpr int CH = 4096, PER = CH / 256;
+  __shared__ uint32_t list[CH];
+  __shared__ uint32_t wsum[256];
+  const InstDev ip = insts[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int ns = ip.nsides;
+  double acc = 0.0;
+  const int s = tid / D, c = tid % D;
+  const int dir = (D == 2) ? ((s < 2) ? 0 : 1) : (s % 3);
+  for (uint32_t base = ip.ckey_begin; base < ip.ckey_end; base += CH) {
+    uint32_t cnt = 0;
+    uint32_t loc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const uint32_t key = base + tid * PER + q;
+      const bool on = key < ip.ckey_end && wflag[key];
+      loc[q] = on ? key : 0xFFFFFFFFu;
+      cnt += on ? 1 : 0;
+    }
+    wsum[tid] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int q = 0; q < 256; ++q) {
+        const uint32_t v = wsum[q];
+        wsum[q] = run;
+        run += v;
+      }
+      list[CH - 1] = run; // total, read back below before being overwritten
+    }
+    __syncthreads();
+    const uint32_t total = list[CH - 1];
+    __syncthreads();
+    uint32_t w = wsum[tid];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+      if (loc[q] != 0xFFFFFFFFu) list[w++] = loc[q];
+    __syncthreads();
+    if (tid < ns * D) {
+      for (uint32_t q = 0; q < total; ++q) {
+        const double *p = wterm + ((size_t)list[q] * 6 + s) * 4;
+        if (c == dir) {
+          acc -= p[0];
+          acc -= p[1];
+        } else if (c == (dir + 1) % D) {
+          acc -= p[2];
+        } else {
+          acc -= p[3];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < ns * D) reaction[(size_t)blockIdx.x * 18 + s * 3 + c] = acc;
+}
+
+// ------------------------------------------------------- neighbour list
+
+// Per-instance box of contact nodes at step t (list build).
+template <int D>
+__global__ __launch_bounds__(256) void k_inst_box(const StepArgs A, const uint32_t t,
+                                                  unsigned long long *ibox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = i < A.n_nodes;
+  uint32_t inst = 0xFFFFFFFFu;
+  double x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = 0.0;
+  if (valid) {
+    if (A.node_info[i] & NI_CONTACT) {
+      const GrainDev gp = A.grains[A.node_grain[i]];
+      const NodeRec r = A.rec[t & 1][i];
+      double Xc[D];
+      cur_pos<D>(r, gp, Xc);
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[k] = Xc[k] + r.u[k];
+      inst = gp.inst;
+    } else {
+      valid = false;
+    }
+  }
+  wave_box_atomic<D>(ibox, inst, valid, x);
+}
+
+template <int D>
+__device__ __forceinline__ void cell_coords(const GridDev &gd, const double *x, int *c) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double f = floor((x[k] - gd.origin[k]) * gd.inv_cell);
+    int v = (f < 0.0) ? 0 : (f > (double)(gd.dims[k] - 1) ? gd.dims[k] - 1 : (int)f);
+    c[k] = v;
+  }
+}
+template <int D> __device__ __forceinline__ uint32_t cell_lin(const GridDev &gd, const int *c) {
+  if (D == 2) return gd.cell_begin + (uint32_t)c[1] * gd.dims[0] + c[0];
+  return gd.cell_begin + ((uint32_t)c[2] * gd.dims[1] + c[1]) * gd.dims[0] + c[0];
+}
+
+template <int D>
+__device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_t i,
+                                          double *x, uint32_t &inst) {
+  if (!(A.node_info[i] & NI_CONTACT)) return false;
+  const GrainDev gp = A.grains[A.node_grain[i]];
+  const NodeRec r = A.rec[t & 1][i];
+  double Xc[D];
+  cur_pos<D>(r, gp, Xc);
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = Xc[k] + r.u[k];
+  inst = gp.inst;
+  return true;
+}
+
+template <int D>
+__global__ void k_cell_count(const StepArgs A, const uint32_t t, const GridDev *grids,
+                             uint32_t *cell_of, uint32_t *cell_cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n_nodes) return;
+  double x[D];
+  uint32_t inst;
+  if (!contact_x<D>(A, t, i, x, inst)) {
+    cell_of[i] = 0xFFFFFFFFu;
+    return;
+  }
+  int c[D];
+  const GridDev gd = grids[inst];
+  cell_coords<D>(gd, x, c);
+  const uint32_t cl = cell_lin<D>(gd, c);
+  cell_of[i] = cl;
+  atomicAdd(&cell_cnt[cl], 1u);
+}
+
+__global__ void k_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t *cell_start,
+                            uint32_t *cell_cur, uint32_t *cell_nodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cl = cell_of[i];
+  if (cl == 0xFFFFFFFFu) return;
+  const uint32_t p = cell_start[cl] + atomicAdd(&cell_cur[cl], 1u);
+  cell_nodes[p] = i;
+}
+
+// Count (FILL=false) or write (FILL=true) the Verlet-list entries of node i:
+// every other contact node of the same instance within r2 at build time.
+template <int D, bool FILL>
+__global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
+                       const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n_nodes) return;
+  double x[D];
+  uint32_t inst;
+  if (!contact_x<D>(A, t, i, x, inst)) {
+    if (!FILL) cnt[i] = 0;
+    return;
+  }
+  const GridDev gd = grids[inst];
+  const double r2 = gd.r2;
+  int c[D];
+  cell_coords<D>(gd, x, c);
+  uint32_t n = 0;
+  const uint32_t out0 = FILL ? ptr[i] : 0;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    lo[k] = c[k] > 0 ? c[k] - 1 : 0;
+    hi[k] = c[k] < gd.dims[k] - 1 ? c[k] + 1 : gd.dims[k] - 1;
+  }
+  int cc[3];
+  for (cc[2] = lo[2]; cc[2] <= hi[2]; ++cc[2])
+    for (cc[1] = lo[1]; cc[1] <= hi[1]; ++cc[1])
+      for (cc[0] = lo[0]; cc[0] <= hi[0]; ++cc[0]) {
+        const uint32_t cl = cell_lin<D>(gd, cc);
+        for (uint32_t p = cell_start[cl]; p < cell_start[cl + 1]; ++p) {
+          const uint32_t j = cell_nodes[p];
+          if (j == i) continue;
+          double y[D];
+          uint32_t ij;
+          contact_x<D>(A, t, j, y, ij);
+          double d[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) d[k] = y[k] - x[k];
+          if (red<D>(d, d) <= r2) {
+            if (FILL) lj[out0 + n] = j;
+            ++n;
+          }
+        }
+      }
+  if (!FILL) {
+    cnt[i] = n;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) xb[(size_t)i * D + k] = x[k];
+  // sort by the reference's accumulation order: own grain first, then by
+  // contact key (grain, then node or boundary-list position)
+  const uint32_t g = A.node_grain[i];
+  for (uint32_t a = 1; a < n; ++a) {
+    const uint32_t v = lj[out0 + a];
+    const unsigned long long kv =
+        ((unsigned long long)(A.node_grain[v] == g ? 0 : 1) << 32) | A.ckey[v];
+    uint32_t b = a;
+    while (b > 0) {
+      const uint32_t u = lj[out0 + b - 1];
+      const unsigned long long ku =
+          ((unsigned long long)(A.node_grain[u] == g ? 0 : 1) << 32) | A.ckey[u];
+      if (ku <= kv) break;
+      lj[out0 + b] = u;
+      --b;
+    }
+    lj[out0 + b] = v;
+  }
+  // bond slot of same-grain partners, for the intact-bond exclusion
+  const uint32_t sl = i >> 6, lane = i & 63;
+  const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+  for (uint32_t a = 0; a < n; ++a) {
+    const uint32_t j = lj[out0 + a];
+    int32_t slot = -1;
+    if (A.node_grain[j] == g) {
+      for (uint32_t k = 0; k < width; ++k) {
+        const uint32_t e = base + 64u * k + lane;
+        if (A.bstat[e] == BOND_PAD) break;
+        if (A.bcol[e] == j) {
+          slot = (int32_t)e;
+          break;
+        }
+      }
+    }
+    lslot[out0 + a] = slot;
+  }
+}
+
+// Contact pairs of the completed step t (readback for parity checks).
+template <int D, bool FILL>
+__global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const uint32_t *ptr,
+                        uint32_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n_nodes) return;
+  uint32_t n = 0;
+  const uint32_t g = A.node_grain[i];
+  const GrainDev gp = A.grains[g];
+  const InstDev &ip = A.insts[gp.inst];
+  const uint8_t info = A.node_info[i];
+  const bool mov = grain_movable(gp, ip, g, t);
+  const bool brk = (ip.flags & IF_FRACTURE) && (gp.flags & GF_BREAKABLE);
+  const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
+  const bool stop = grain_stoppable(gp, ip, g, t);
+  if (mov && (info & NI_CONTACT) && (selfc || stop)) {
+    const NodeRec *Rr = A.rec[t & 1];
+    const NodeRec ri = Rr[i];
+    double Xc[D], xc[D], box[2 * D];
+    cur_pos<D>(ri, gp, Xc);
+#pragma unroll
+    for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ri.u[k];
+    load_box<D>(A.bbox[t % 3], g, box);
+    const uint32_t o = FILL ? ptr[i] : 0;
+    for (uint32_t c = A.cl_ptr[i]; c < A.cl_ptr[i + 1]; ++c) {
+      uint32_t j;
+      double d[D], r, R, K;
+      if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, box, Rr, j, d, r, R, K))
+        continue;
+      if (FILL) {
+        out[2 * (o + n)] = i;
+        out[2 * (o + n) + 1] = j;
+      }
+      ++n;
+    }
+  }
+  if (!FILL) cnt[i] = n;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long a,
+                           unsigned long long b, uint32_t period, uint32_t split) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = ((i % period) < split) ? a : b;
+}
+
+// ------------------------------------------------------------- launchers
+
+static inline uint32_t nblk(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
+hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int block,
+                       hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, block) > 0 ? nblk(A.n_nodes, block) : 1;
+  if (dim == 2) {
+    if (record) hipLaunchKernelGGL((k_step<2, true>), dim3(nb), dim3(block), 0, s, A, t);
+    else hipLaunchKernelGGL((k_step<2, false>), dim3(nb), dim3(block), 0, s, A, t);
+  } else {
+    if (record) hipLaunchKernelGGL((k_step<3, true>), dim3(nb), dim3(block), 0, s, A, t);
+    else hipLaunchKernelGGL((k_step<3, false>), dim3(nb), dim3(block), 0, s, A, t);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
+  if (dim == 2) hipLaunchKernelGGL(k_prologue<2>, dim3(nb), dim3(256), 0, s, A);
+  else hipLaunchKernelGGL(k_prologue<3>, dim3(nb), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gmean,
+                             hipStream_t s) {
+  const uint32_t nb = nblk(A.n_grains, 64);
+  if (dim == 2) hipLaunchKernelGGL(k_grain_mean<2>, dim3(nb), dim3(64), 0, s, A, t, gmean);
+  else hipLaunchKernelGGL(k_grain_mean<3>, dim3(nb), dim3(64), 0, s, A, t, gmean);
+  return hipGetLastError();
+}
+
+hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
+                           const double *wterm, const uint8_t *wflag, double *reaction,
+                           hipStream_t s) {
+  if (dim == 2)
+    hipLaunchKernelGGL(k_reaction<2>, dim3(n_insts), dim3(256), 0, s, insts, wterm, wflag,
+                       reaction);
+  else
+    hipLaunchKernelGGL(k_reaction<3>, dim3(n_insts), dim3(256), 0, s, insts, wterm, wflag,
+                       reaction);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_box(unsigned long long *p, size_t n, int dim, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, ~0ull, 0ull,
+                     (uint32_t)(2 * dim), (uint32_t)dim);
+  return hipGetLastError();
+}
+
+hipError_t launch_inst_box(int dim, const StepArgs &A, uint32_t t, unsigned long long *ibox,
+                           hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
+  if (dim == 2) hipLaunchKernelGGL(k_inst_box<2>, dim3(nb), dim3(256), 0, s, A, t, ibox);
+  else hipLaunchKernelGGL(k_inst_box<3>, dim3(nb), dim3(256), 0, s, A, t, ibox);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_count(int dim, const StepArgs &A, uint32_t t, const GridDev *grids,
+                             uint32_t *cell_of, uint32_t *cell_cnt, hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 256);
+  if (nb == 0) return hipSuccess;
+  if (dim == 2)
+    hipLaunchKernelGGL(k_cell_count<2>, dim3(nb), dim3(256), 0, s, A, t, grids, cell_of, cell_cnt);
+  else
+    hipLaunchKernelGGL(k_cell_count<3>, dim3(nb), dim3(256), 0, s, A, t, grids, cell_of, cell_cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t *cell_start,
+                            uint32_t *cell_cur, uint32_t *cell_nodes, hipStream_t s) {
+  const uint32_t nb = nblk(n, 256);
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, s, n, cell_of, cell_start, cell_cur,
+                     cell_nodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_list(int dim, bool fill, const StepArgs &A, uint32_t t, const GridDev *grids,
+                       const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb,
+                       hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 128);
+  if (nb == 0) return hipSuccess;
+  if (dim == 2) {
+    if (fill)
+      hipLaunchKernelGGL((k_list<2, true>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
+                         cell_nodes, cnt, ptr, lj, lslot, xb);
+    else
+      hipLaunchKernelGGL((k_list<2, false>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
+                         cell_nodes, cnt, ptr, lj, lslot, xb);
+  } else {
+    if (fill)
+      hipLaunchKernelGGL((k_list<3, true>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
+                         cell_nodes, cnt, ptr, lj, lslot, xb);
+    else
+      hipLaunchKernelGGL((k_list<3, false>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
+                         cell_nodes, cnt, ptr, lj, lslot, xb);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(int dim, bool fill, const StepArgs &A, uint32_t t, uint32_t *cnt,
+                        const uint32_t *ptr, uint32_t *out, hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 128);
+  if (nb == 0) return hipSuccess;
+  if (dim == 2) {
+    if (fill) hipLaunchKernelGGL((k_pairs<2, true>), dim3(nb), dim3(128), 0, s, A, t, cnt, ptr, out);
+    else hipLaunchKernelGGL((k_pairs<2, false>), dim3(nb), dim3(128), 0, s, A, t, cnt, ptr, out);
+  } else {
+    if (fill) hipLaunchKernelGGL((k_pairs<3, true>), dim3(nb), dim3(128), 0, s, A, t, cnt, ptr, out);
+    else hipLaunchKernelGGL((k_pairs<3, false>), dim3(nb), dim3(128), 0, s, A, t, cnt, ptr, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp,
+                              size_t *tmp_bytes, hipStream_t s) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, s);
+}
+
+} // namespace pg
+
+namespace pg {
+
+// Diagnostics: the division and square root the step kernels use, applied
+// elementwise, so tests can compare their rounding with the host's.
+__global__ void k_arith(uint64_t n, const double *a, const double *b, double *q, double *r) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  q[i] = a[i] / b[i];
+  r[i] = sqrt(a[i]);
+}
+
+hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
+                        hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_arith, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, a, b, q, r);
+  return hipGetLastError();
+}
+
+} // namespace pg

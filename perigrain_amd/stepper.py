@@ -1,0 +1,137 @@
+"""Python driver of libpg: the HIP time stepper behind the C ABI.
+
+`Stepper` mirrors run_timeloop's role (src/particle/timeloop.h:500-1015):
+it owns the packed device state of one simulation (or a batch) and advances
+it step by step; snapshots are read back in the reference's layouts.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, lib
+
+
+class Stepper:
+    """One simulation (or a batch of independent ones) on a HIP device."""
+
+    def __init__(self, setups, params, device=0):
+        single = not isinstance(setups, (list, tuple))
+        if single:
+            setups, params = [setups], [params]
+        if len(setups) != len(params):
+            raise ValueError("setups and params must have the same length")
+        L = lib()
+        self._structs = []
+        n = len(setups)
+        sarr = (_ffi.PgSetup * n)()
+        parr = (_ffi.PgParams * n)()
+        for k, (s, p) in enumerate(zip(setups, params)):
+            st, keep = _ffi.setup_struct(s)
+            self._structs.append(keep)
+            sarr[k] = st
+            parr[k] = _ffi.params_struct(p)
+        h = ctypes.c_void_p()
+        check(L.pg_batch_create(sarr, parr, n, int(device), ctypes.byref(h)))
+        self._h = h
+        self.dim = int(setups[0].dim)
+        self.n_instances = n
+        self.setups = setups
+        self.params = params
+
+    # ------------------------------------------------------------- control
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n=1):
+        check(lib().pg_step(self._h, int(n)))
+
+    def sync(self):
+        check(lib().pg_sync(self._h))
+
+    @property
+    def t(self):
+        v = ctypes.c_uint32()
+        check(lib().pg_current_step(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def sizes(self, inst=0):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().pg_sizes(self._h, inst, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    # ------------------------------------------------------------ readback
+    def read(self, field, inst=0):
+        n, _, _ = self.sizes(inst)
+        out = np.empty((n, self.dim), dtype=np.float64)
+        check(lib().pg_read_nodes(self._h, inst, _ffi.FIELDS[field],
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def wall(self, inst=0):
+        info = np.zeros(6)
+        react = np.zeros(6 * self.dim)
+        check(lib().pg_read_wall(self._h, inst, info.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 react.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        ns = 4 if self.dim == 2 else 6
+        return info[:ns].copy(), react.reshape(6, self.dim)[:ns].copy()
+
+    def bond_intact(self, inst=0):
+        _, _, ns = self.sizes(inst)
+        out = np.empty(ns, dtype=np.uint8)
+        check(lib().pg_read_bond_intact(self._h, inst,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def conn(self, grain, inst=0):
+        rows = ctypes.c_uint64()
+        check(lib().pg_read_conn(self._h, inst, grain, None, 0, ctypes.byref(rows)))
+        out = np.empty((rows.value, 2), dtype=np.uint32)
+        check(lib().pg_read_conn(self._h, inst, grain,
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                 rows.value, ctypes.byref(rows)))
+        return out
+
+    def contacts(self, inst=0):
+        n = ctypes.c_uint64()
+        check(lib().pg_read_contacts(self._h, inst, None, 0, ctypes.byref(n)))
+        out = np.empty((n.value, 2), dtype=np.uint32)
+        if n.value:
+            check(lib().pg_read_contacts(self._h, inst,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                         n.value, ctypes.byref(n)))
+        return out
+
+    def stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().pg_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"rebuilds": a.value, "list_entries": b.value, "skipped_steps": c.value}
+
+    def set_tuning(self, skin_factor=0.5, block=256):
+        check(lib().pg_set_tuning(self._h, float(skin_factor), int(block)))
+
+    def set_graph(self, on=True):
+        check(lib().pg_set_graph(self._h, 1 if on else 0))
+
+    # -------------------------------------------------------------- timing
+    def event(self, slot):
+        check(lib().pg_event_record(self._h, slot))
+
+    def elapsed_ms(self, a, b):
+        ms = ctypes.c_float()
+        check(lib().pg_event_elapsed(self._h, a, b, ctypes.byref(ms)))
+        return float(ms.value)

@@ -1,0 +1,146 @@
+"""GPU parity: the HIP stepper (through the libpg C ABI) against the oracle.
+
+Bit-exact: bond status, broken-bond sets, contact pair lists, and every
+FP64 field where no libm transcendental is involved (2D walls call asin,
+whose last bit may differ between the host libm and the device's).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from perigrain_amd import Stepper, _ffi, setups
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("CurrPos", "vel", "acc", "force")
+
+
+def _both(s, p, threads):
+    g = Stepper(s, p, device=0)
+    o = oracle.Oracle(s, p, threads=threads)
+    return g, o
+
+
+def _advance(g, o, n):
+    g.step(n)
+    o.step(n, record_contacts=True)
+
+
+def _assert_exact(g, o, what=""):
+    for f in FIELDS:
+        a, b = g.read(f), o.read(f)
+        bad = np.argwhere(a != b)
+        assert len(bad) == 0, f"{what} {f}: {len(bad)} entries differ, first {bad[:3].tolist()} " \
+                              f"gpu={a[tuple(bad[0])]!r} oracle={b[tuple(bad[0])]!r}"
+    assert np.array_equal(g.bond_intact(), o.bond_intact()), f"{what} bond status differs"
+    assert np.array_equal(g.contacts(), o.contacts()), f"{what} contact pairs differ"
+    wg, rg = g.wall()
+    wo, ro = o.wall()
+    assert np.array_equal(wg, wo), f"{what} wall positions differ"
+    assert np.array_equal(rg, ro), f"{what} wall reaction differs: {rg} vs {ro}"
+
+
+def _assert_close(g, o, rtol, what=""):
+    for f in FIELDS:
+        a, b = g.read(f), o.read(f)
+        scale = max(np.max(np.abs(b)), 1e-300)
+        err = np.max(np.abs(a - b)) / scale
+        assert err <= rtol, f"{what} {f}: max error {err:.3e} (relative to max |x|) > {rtol}"
+    assert np.array_equal(g.bond_intact(), o.bond_intact()), f"{what} bond status differs"
+    assert np.array_equal(g.contacts(), o.contacts()), f"{what} contact pairs differ"
+    wg, rg = g.wall()
+    wo, ro = o.wall()
+    assert np.array_equal(wg, wo), f"{what} wall positions differ"
+    scale = max(np.max(np.abs(ro)), 1e-300)
+    assert np.max(np.abs(rg - ro)) / scale <= rtol, f"{what} wall reaction {rg} vs {ro}"
+
+
+def test_device_present():
+    assert _ffi.device_count() >= 1
+
+
+def test_division_and_sqrt_round_like_host():
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    a = np.concatenate([rng.uniform(1e-12, 1e-6, n), 10.0 ** rng.uniform(-30, 30, n),
+                        rng.uniform(0.5, 2.0, n)])
+    b = np.concatenate([rng.uniform(1e-6, 1e-3, n), 10.0 ** rng.uniform(-30, 30, n),
+                        rng.uniform(0.5, 2.0, n)])
+    q = np.empty_like(a)
+    r = np.empty_like(a)
+    dp = lambda x: x.ctypes.data_as(_ffi.ctypes.POINTER(_ffi.ctypes.c_double))
+    _ffi.check(_ffi.lib().pg_check_arith(0, len(a), dp(a), dp(b), dp(q), dp(r)))
+    assert np.array_equal(q, a / b)
+    assert np.array_equal(r, np.sqrt(a))
+
+
+def test_two_disk_collision_bit_exact(oracle_threads):
+    s, p = setups.two_disk_collision(seed=3, gap=5e-5)
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1, 50, 200, 400):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    assert len(o.contacts()) > 0, "workload should be in contact by t=400"
+
+
+def test_two_disk_fracture_bit_exact(oracle_threads):
+    s, p = setups.two_disk_collision(seed=4, gap=1e-5, speed=50.0)
+    # disk 0 expands radially: its bonds stretch past s0 within ~30 steps
+    nodes = s.grain_nodes(0)
+    s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (20, 100, 250):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    assert (o.bond_intact() == 0).sum() > 0, "workload should break bonds"
+
+
+def test_3d_compression_with_walls_bit_exact(oracle_threads):
+    s, p = setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-5.0)
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1, 60, 150):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    _, r = o.wall()
+    assert np.any(r != 0), "top wall should push"
+
+
+def test_2d_settle_with_walls_close(oracle_threads):
+    s, p = setups.bulk_settle_2d(seed=6, n_grains=16, box=(8e-3, 16e-3), nodes_per_grain=80)
+    # start the grains at 100 m/s in random directions so that they collide
+    # with each other and with the walls within a few hundred steps
+    rng = np.random.default_rng(0)
+    for gi in range(s.n_grains):
+        a = rng.uniform(0, 2 * np.pi)
+        s.vel[s.grain_nodes(gi)] = 100.0 * np.array([np.cos(a), np.sin(a)])
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1, 200, 600):
+        _advance(g, o, t - done)
+        done = t
+        _assert_close(g, o, 1e-10, f"t={t}")
+
+
+def test_batched_equals_single():
+    sp = [setups.settle_instance_2d(seed) for seed in (11, 12, 13)]
+    ss = [setups.subset(s, list(range(min(12, s.n_grains)))) for s, _ in sp]
+    ps = [p for _, p in sp]
+    steps = 120
+    with Stepper(ss, ps, device=0) as b:
+        b.step(steps)
+        for k, (s, p) in enumerate(zip(ss, ps)):
+            with Stepper(s, p, device=0) as one:
+                one.step(steps)
+                for f in FIELDS:
+                    assert np.array_equal(b.read(f, inst=k), one.read(f)), f"instance {k} {f}"
+                assert np.array_equal(b.bond_intact(inst=k), one.bond_intact())
+                assert np.array_equal(b.contacts(inst=k), one.contacts())
+                wb, rb = b.wall(inst=k)
+                w1, r1 = one.wall()
+                assert np.array_equal(wb, w1) and np.array_equal(rb, r1)
