@@ -64,17 +64,18 @@ struct pg_handle {
   std::vector<InstDev> inst_dev;
   std::vector<uint32_t> deg, slice_base, slice_width, bcol, grain_node_begin;
   std::vector<uint64_t> slot0;
+  std::vector<uint32_t> perm, inv; // device index <-> reference index
+  bool reorder = true;
   std::vector<void *> allocs;
   // device
   NodeRec *rec[2] = {nullptr, nullptr};
   double *vel[2] = {nullptr, nullptr};
   double *acc = nullptr, *ext = nullptr, *xb = nullptr, *gmean = nullptr;
-  uint32_t *node_grain = nullptr, *ckey = nullptr;
+  uint32_t *node_grain = nullptr, *ckey = nullptr, *ext2int = nullptr;
   uint8_t *node_info = nullptr;
   GrainDev *grains = nullptr;
   InstDev *insts = nullptr;
   uint32_t *d_slice_base = nullptr, *d_slice_width = nullptr, *d_bcol = nullptr;
-  uint8_t *bstat = nullptr;
   uint32_t *cl_ptr = nullptr, *cl_j = nullptr;
   int32_t *cl_slot = nullptr;
   size_t cl_cap = 0;
@@ -150,12 +151,12 @@ void fill_args(pg_handle *h) {
   A.node_grain = h->node_grain;
   A.node_info = h->node_info;
   A.ckey = h->ckey;
+  A.ext2int = h->ext2int;
   A.grains = h->grains;
   A.insts = h->insts;
   A.slice_base = h->d_slice_base;
   A.slice_width = h->d_slice_width;
   A.bcol = h->d_bcol;
-  A.bstat = h->bstat;
   A.cl_ptr = h->cl_ptr;
   A.cl_j = h->cl_j;
   A.cl_slot = h->cl_slot;
@@ -313,7 +314,7 @@ int check_setup(const pg_setup *s, const pg_params *p, int dim_expected) {
         return fail(PG_ERR_INVALID, "torque_axis must be 2 in 2D and < 3 in 3D");
     }
   }
-  if (s->node_offset[s->n_grains] >= 0xFFFFFF00ll)
+  if (s->node_offset[s->n_grains] >= 0x7FFFFF00ll)
     return fail(PG_ERR_INVALID, "too many nodes");
   if (!(p->dt > 0)) return fail(PG_ERR_INVALID, "dt must be > 0");
   if (p->reset_partzero_y)
@@ -334,11 +335,9 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   for (int s = 0; s < n; ++s) RC(check_setup(&setups[s], &params[s], D));
   h->dim = D;
   h->device = device;
-  HIPCHK(hipSetDevice(device));
-  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
 
-  // ---- sizes and per-instance ranges
+  // ---- sizes and per-instance ranges (host-side validation and staging
+  // come first, so a bad setup is reported before any device work)
   uint64_t N = 0, G = 0, CK = 0, slots = 0;
   h->inst.resize(n);
   for (int s = 0; s < n; ++s) {
@@ -355,7 +354,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
     ih.nsides = D == 2 ? 4 : 6;
   }
-  if (N >= 0xFFFFFF00ull) return fail(PG_ERR_INVALID, "too many nodes in batch");
+  if (N >= 0x7FFFFF00ull) return fail(PG_ERR_INVALID, "too many nodes in batch");
   h->N = (uint32_t)N;
   h->G = (uint32_t)G;
   h->I = (uint32_t)n;
@@ -516,7 +515,37 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     h->inst[s].slot_begin = row_ptr[h->inst[s].node_begin];
     h->inst[s].slot_end = row_ptr[h->inst[s].node_end];
   }
-  // ---- sliced ELL
+  // ---- device node order: within each grain, nodes sorted by degree
+  // (descending, stable), so the 64 rows of a slice have nearly equal
+  // lengths.  perm maps device index -> reference index, inv the reverse.
+  // Everything order-sensitive (row order, contact order keys, the torque
+  // mean) keeps the reference order.
+  std::vector<uint32_t> &perm = h->perm, &inv = h->inv;
+  perm.resize(N);
+  inv.resize(N);
+  for (uint32_t gg = 0; gg < G; ++gg) {
+    const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
+    for (uint32_t q = a; q < b; ++q) perm[q] = q;
+    if (h->reorder)
+      std::stable_sort(perm.begin() + a, perm.begin() + b,
+                       [&](uint32_t x, uint32_t y) { return h->deg[x] > h->deg[y]; });
+  }
+  for (uint64_t i = 0; i < N; ++i) inv[perm[i]] = (uint32_t)i;
+  auto permute = [&](auto &v, size_t w) {
+    auto out = v;
+    for (uint64_t i = 0; i < N; ++i)
+      for (size_t k = 0; k < w; ++k) out[i * w + k] = v[(uint64_t)perm[i] * w + k];
+    v.swap(out);
+  };
+  permute(rec, 1);
+  permute(vel, D);
+  permute(acc, D);
+  permute(ext, D);
+  permute(node_grain, 1);
+  permute(node_info, 1);
+  permute(ckey, 1);
+
+  // ---- sliced ELL over the device order
   const uint32_t S = (uint32_t)((N + SLICE - 1) / SLICE);
   h->S = S;
   h->slice_base.assign(S, 0);
@@ -525,7 +554,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   for (uint32_t sl = 0; sl < S; ++sl) {
     uint32_t w = 0;
     for (uint64_t i = (uint64_t)sl * SLICE; i < std::min<uint64_t>(N, (uint64_t)(sl + 1) * SLICE); ++i)
-      w = std::max(w, h->deg[i]);
+      w = std::max(w, h->deg[perm[i]]);
     h->slice_base[sl] = (uint32_t)E;
     h->slice_width[sl] = w;
     E += (uint64_t)w * SLICE;
@@ -533,17 +562,21 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   }
   h->E = E;
   h->bcol.assign(E, 0);
-  std::vector<uint8_t> bstat(E, BOND_PAD);
+  std::vector<uint32_t> bcol_init(E + (uint64_t)SLICE * BCOL_TAIL, BCOL_PAD);
   for (uint64_t i = 0; i < N; ++i) {
     const uint32_t sl = (uint32_t)(i / SLICE), lane = (uint32_t)(i % SLICE);
-    for (uint32_t k = 0; k < h->deg[i]; ++k) {
+    const uint32_t r = perm[i];
+    for (uint32_t k = 0; k < h->deg[r]; ++k) {
       const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
-      h->bcol[e] = rows[row_ptr[i] + k];
-      bstat[e] = BOND_INTACT;
+      h->bcol[e] = inv[rows[row_ptr[r] + k]];
+      bcol_init[e] = h->bcol[e];
     }
   }
 
   // ---- device allocations
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
   const size_t ND = (size_t)N * D;
   RC(dalloc(h, h->rec[0], N));
   RC(dalloc(h, h->rec[1], N));
@@ -556,12 +589,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->node_grain, N));
   RC(dalloc(h, h->node_info, N));
   RC(dalloc(h, h->ckey, N));
+  RC(dalloc(h, h->ext2int, N));
   RC(dalloc(h, h->grains, G));
   RC(dalloc(h, h->insts, n));
   RC(dalloc(h, h->d_slice_base, S));
   RC(dalloc(h, h->d_slice_width, S));
-  RC(dalloc(h, h->d_bcol, E));
-  RC(dalloc(h, h->bstat, E));
+  RC(dalloc(h, h->d_bcol, E + (size_t)SLICE * BCOL_TAIL));
   RC(dalloc(h, h->cl_ptr, (size_t)N + 1));
   RC(dalloc(h, h->cl_cnt, (size_t)N + 1));
   h->cl_cap = 1024;
@@ -595,12 +628,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(h2d(h, h->node_grain, node_grain.data(), N));
   RC(h2d(h, h->node_info, node_info.data(), N));
   RC(h2d(h, h->ckey, ckey.data(), N));
+  RC(h2d(h, h->ext2int, h->inv.data(), N));
   RC(h2d(h, h->grains, grains.data(), G));
   RC(h2d(h, h->insts, h->inst_dev.data(), (size_t)n));
   RC(h2d(h, h->d_slice_base, h->slice_base.data(), S));
   RC(h2d(h, h->d_slice_width, h->slice_width.data(), S));
-  RC(h2d(h, h->d_bcol, h->bcol.data(), E));
-  RC(h2d(h, h->bstat, bstat.data(), E));
+  RC(h2d(h, h->d_bcol, bcol_init.data(), bcol_init.size()));
   std::vector<double> walls((size_t)n * 6);
   for (int s = 0; s < n; ++s)
     for (int k = 0; k < 6; ++k) walls[(size_t)s * 6 + k] = setups[s].wall[k];
@@ -746,25 +779,29 @@ int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
   const InstHost &ih = h->inst[inst];
   const size_t n = ih.node_end - ih.node_begin;
   HIPCHK(hipSetDevice(h->device));
-  switch (field) {
-    case PG_FIELD_CURRPOS:
-    case PG_FIELD_DISP: {
-      std::vector<NodeRec> r(n);
-      RC(d2h(h, r.data(), h->rec[h->t & 1] + ih.node_begin, n));
-      for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < D; ++k)
-          dst[i * D + k] = field == PG_FIELD_DISP ? r[i].u[k] : r[i].X[k] + r[i].u[k];
-      return PG_OK;
+  // device order -> reference order (the permutation stays inside the range)
+  auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
+  if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
+    std::vector<NodeRec> r(n);
+    RC(d2h(h, r.data(), h->rec[h->t & 1] + ih.node_begin, n));
+    for (size_t e = 0; e < n; ++e) {
+      const NodeRec &q = r[at(e)];
+      for (int k = 0; k < D; ++k) dst[e * D + k] = field == PG_FIELD_DISP ? q.u[k] : q.X[k] + q.u[k];
     }
-    case PG_FIELD_VEL:
-      return d2h(h, dst, h->vel[h->t & 1] + (size_t)ih.node_begin * D, n * D);
-    case PG_FIELD_ACC:
-      return d2h(h, dst, h->acc + (size_t)ih.node_begin * D, n * D);
-    case PG_FIELD_FORCE:
-      return d2h(h, dst, h->force_out + (size_t)ih.node_begin * D, n * D);
-    default:
-      return fail(PG_ERR_INVALID, "unknown field");
+    return PG_OK;
   }
+  const double *src = nullptr;
+  switch (field) {
+    case PG_FIELD_VEL: src = h->vel[h->t & 1]; break;
+    case PG_FIELD_ACC: src = h->acc; break;
+    case PG_FIELD_FORCE: src = h->force_out; break;
+    default: return fail(PG_ERR_INVALID, "unknown field");
+  }
+  std::vector<double> v(n * D);
+  RC(d2h(h, v.data(), src + (size_t)ih.node_begin * D, n * D));
+  for (size_t e = 0; e < n; ++e)
+    for (int k = 0; k < D; ++k) dst[e * D + k] = v[at(e) * D + k];
+  return PG_OK;
 }
 
 int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
@@ -782,14 +819,15 @@ int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
 
 int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst) {
   if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  std::vector<uint8_t> st(h->E);
-  RC(d2h(h, st.data(), h->bstat, h->E));
+  std::vector<uint32_t> st(h->E);
+  RC(d2h(h, st.data(), h->d_bcol, h->E));
   const InstHost &ih = h->inst[inst];
-  for (uint32_t i = ih.node_begin; i < ih.node_end; ++i) {
+  for (uint32_t e = ih.node_begin; e < ih.node_end; ++e) { // reference order
+    const uint32_t i = h->inv[e];
     const uint32_t sl = i / SLICE, lane = i % SLICE;
-    for (uint32_t k = 0; k < h->deg[i]; ++k)
-      dst[h->slot0[i] - ih.slot_begin + k] =
-          st[h->slice_base[sl] + (uint64_t)SLICE * k + lane] == BOND_INTACT ? 1 : 0;
+    for (uint32_t k = 0; k < h->deg[e]; ++k)
+      dst[h->slot0[e] - ih.slot_begin + k] =
+          (st[h->slice_base[sl] + (uint64_t)SLICE * k + lane] & BCOL_BROKEN) ? 0 : 1;
   }
   return PG_OK;
 }
@@ -803,15 +841,16 @@ int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_
   const uint32_t gg = ih.grain_begin + grain;
   const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
   // status of this grain's rows only
-  std::vector<uint8_t> st(h->E);
-  RC(d2h(h, st.data(), h->bstat, h->E));
+  std::vector<uint32_t> st(h->E);
+  RC(d2h(h, st.data(), h->d_bcol, h->E));
   uint64_t r = 0;
-  for (uint32_t i = a; i < b; ++i) {
+  for (uint32_t ge = a; ge < b; ++ge) { // reference node order
+    const uint32_t i = h->inv[ge];
     const uint32_t sl = i / SLICE, lane = i % SLICE;
-    for (uint32_t k = 0; k < h->deg[i]; ++k) {
+    for (uint32_t k = 0; k < h->deg[ge]; ++k) {
       const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
-      if (st[e] != BOND_INTACT) continue;
-      const uint32_t q = h->bcol[e] - a, li = i - a;
+      if (st[e] & BCOL_BROKEN) continue;
+      const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
       if (li < q) {
         if (dst && r < cap_rows) {
           dst[2 * r] = li;
@@ -853,16 +892,22 @@ int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs, 
   dfree(h, out);
   if (rc) return rc;
   const InstHost &ih = h->inst[inst];
-  uint64_t np = 0;
+  // to reference indices, ordered by node and then by accumulation order
+  std::vector<std::pair<uint32_t, uint32_t>> pr;
   for (uint64_t q = 0; q < total; ++q) {
-    const uint32_t i = all[2 * q], j = all[2 * q + 1];
+    const uint32_t i = h->perm[all[2 * q]], j = h->perm[all[2 * q + 1]];
     if (i < ih.node_begin || i >= ih.node_end) continue;
-    if (dst && np < cap_pairs) {
-      dst[2 * np] = i - ih.node_begin;
-      dst[2 * np + 1] = j - ih.node_begin;
-    }
-    ++np;
+    pr.push_back({i - ih.node_begin, j - ih.node_begin});
   }
+  std::stable_sort(pr.begin(), pr.end(),
+                   [](const std::pair<uint32_t, uint32_t> &x,
+                      const std::pair<uint32_t, uint32_t> &y) { return x.first < y.first; });
+  const uint64_t np = pr.size();
+  if (dst)
+    for (uint64_t q = 0; q < std::min(np, cap_pairs); ++q) {
+      dst[2 * q] = pr[q].first;
+      dst[2 * q + 1] = pr[q].second;
+    }
   if (pairs) *pairs = np;
   return PG_OK;
 }

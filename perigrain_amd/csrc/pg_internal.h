@@ -25,11 +25,14 @@
 
 namespace pg {
 
-constexpr uint8_t BOND_INTACT = 0;
-constexpr uint8_t BOND_BROKEN = 1;
-constexpr uint8_t BOND_PAD = 0xFF;
+// Bond slots hold one 32-bit column word: the partner's node index, with the
+// top bit set once the bond has broken; padding slots are all ones.
+constexpr uint32_t BCOL_BROKEN = 0x80000000u;
+constexpr uint32_t BCOL_INDEX = 0x7FFFFFFFu;
+constexpr uint32_t BCOL_PAD = 0xFFFFFFFFu;
 
 constexpr int SLICE = 64; // rows per sliced-ELL slice (= wavefront width)
+constexpr int BCOL_TAIL = 8; // padding slots past the last slice (bond-loop read-ahead)
 
 // grain flag bits
 constexpr int32_t GF_MOVABLE = 1;
@@ -51,7 +54,7 @@ constexpr int32_t IF_TORQUE = 128;
 constexpr uint8_t NI_CLAMPED = 1;
 constexpr uint8_t NI_CONTACT = 2; // takes part in node-node and wall contact
 
-struct NodeRec {
+struct alignas(64) NodeRec {
   double X[4]; // pos[0..dim-1]; X[dim] = vol (3D: X[3]; 2D: X[2], X[3] = 0)
   double u[4]; // disp[0..dim-1]
 };
@@ -102,12 +105,12 @@ struct StepArgs {
   const uint32_t *node_grain;
   const uint8_t *node_info;
   const uint32_t *ckey;
+  const uint32_t *ext2int; // reference node index -> device node index
   const GrainDev *grains;
   const InstDev *insts;
   const uint32_t *slice_base;
   const uint32_t *slice_width;
-  const uint32_t *bcol;
-  uint8_t *bstat;
+  uint32_t *bcol; // column words (partner index | BCOL_BROKEN), BCOL_PAD
   const uint32_t *cl_ptr;
   const uint32_t *cl_j;
   const int32_t *cl_slot;

@@ -15,6 +15,12 @@
 #include "pg_internal.h"
 #include "pg_kernels.h"
 
+// Waves per SIMD the step kernel is compiled for: 4 caps it at 128 VGPRs,
+// which the bond loop fits without spilling.
+#ifndef PG_STEP_WAVES_PER_EU
+#define PG_STEP_WAVES_PER_EU 4
+#endif
+
 namespace pg {
 
 // ---------------------------------------------------------------- helpers
@@ -213,6 +219,54 @@ __device__ __forceinline__ void pair_force(const double *d, double r, double R, 
   }
 }
 
+// Partner data a bond needs: reference position, volume, displacement.
+template <int D> struct Nbr {
+  double X[D];
+  double V;
+  double u[D];
+};
+
+// Load the partner record of a column word.  Broken and padding slots load
+// the node's own record instead (always a valid address), so the bond loop
+// has no branches around its loads.
+template <int D>
+__device__ __forceinline__ void load_nbr(const NodeRec *R, uint32_t cw, uint32_t self,
+                                         Nbr<D> &nb) {
+  const NodeRec &r = R[(cw & BCOL_BROKEN) ? self : cw];
+#pragma unroll
+  for (int k = 0; k < D; ++k) nb.X[k] = r.X[k];
+  nb.V = r.X[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) nb.u[k] = r.u[k];
+}
+
+// One bond of get_peridynamic_force (particle2.h:125-149) and its break test
+// (remove_bonds, particle2.h:207), evaluated for every slot; broken and
+// padding slots add exactly +0.0 (the sum starts at +0.0 and can never
+// become -0.0, so adding +0.0 leaves it bit-identical).  A bond whose
+// stretch exceeds s0 gets the broken flag in its column word.
+template <int D>
+__device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const double *Xo,
+                                          const double *ui, double cnot, double snot, bool brk,
+                                          uint32_t *slot, double *F) {
+  const bool live = !(cw & BCOL_BROKEN);
+  double xi[D], eta[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) xi[q] = nb.X[q] - Xo[q];
+  const double xin = sqrt(red<D>(xi, xi));
+#pragma unroll
+  for (int q = 0; q < D; ++q) eta[q] = (nb.u[q] - ui[q]) + xi[q];
+  const double n = sqrt(red<D>(eta, eta));
+  const double s = (n - xin) / xin;
+  const double w = (cnot * s) * nb.V;
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    const double term = w * (eta[q] / n);
+    F[q] = F[q] + (live ? term : 0.0);
+  }
+  if (live && brk && s > snot) *slot = cw | BCOL_BROKEN;
+}
+
 // Contact partner test shared by the step kernel and the pair readback:
 // returns true and fills (d, r, R, K) if partner list entry c of node i is
 // in contact at step t (the gates of timeloop.h:698-747 and the self-contact
@@ -234,7 +288,7 @@ __device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, u
   if (gj == g) {
     if (!selfc) return false;
     const int32_t bs = A.cl_slot[c];
-    if (bs >= 0 && A.bstat[bs] == BOND_INTACT) return false;
+    if (bs >= 0 && !(A.bcol[bs] & BCOL_BROKEN)) return false; // intact bond
     cur_pos<D>(rj, gp, Xj);
     double dX[D];
 #pragma unroll
@@ -315,7 +369,8 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 // the Verlet-skin check.  Reads u^t from rec[t&1] and v^{t-1} from
 // vel[(t+1)&1]; writes u^{t+1} to rec[(t+1)&1] and v^t to vel[t&1].
 template <int D, bool RECORD>
-__global__ __launch_bounds__(256) void k_step(const StepArgs A, const uint32_t t) {
+__global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
+                                                                   const uint32_t t) {
   if (*A.violation <= t) return; // neighbour list stale since an earlier step
   const uint32_t nb = gridDim.x;
   const uint32_t lb = xcd_block(blockIdx.x, nb);
@@ -341,16 +396,73 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs A, const uint32_t t
 
   if (valid) {
     g = A.node_grain[i];
+    const NodeRec *Rr = A.rec[t & 1];
+    double F[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) F[k] = 0.0;
+
+    // ---- (a) PMB bond force with bond breaking.  Only what the bond loop
+    // needs is live here; everything else is loaded after it.  The loop is
+    // software-pipelined: the partner records of the next two bonds and the
+    // column words of the two after those are in flight while two bonds are
+    // evaluated.
+    {
+      const GrainDev *gq = A.grains + g;
+      const int32_t gflags = gq->flags;
+      const InstDev &iq = A.insts[gq->inst];
+      const bool mov_a = (gflags & GF_MOVABLE) ||
+                         (iq.set_movable_grain == (int32_t)g && t >= iq.set_movable_step);
+      if (mov_a) {
+        double Xo[D], ui[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          Xo[k] = Rr[i].X[k];
+          ui[k] = Rr[i].u[k];
+        }
+        const double cnot = gq->cnot, snot = gq->snot;
+        const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
+        const uint32_t sl = i >> 6, lane = i & 63;
+      const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+      uint32_t *bc = A.bcol + base + lane;
+      // the column array has BCOL_TAIL spare slices after the last one, so
+      // reading up to 8 slots past a row's width stays in bounds
+      auto col = [&](uint32_t k) -> uint32_t {
+        const uint32_t w = bc[64u * k];
+        return k < width ? w : BCOL_PAD;
+      };
+      Nbr<D> ra0, ra1, rb0, rb1;
+      uint32_t ca0 = col(0), ca1 = col(1);
+      load_nbr<D>(Rr, ca0, i, ra0);
+      load_nbr<D>(Rr, ca1, i, ra1);
+      uint32_t cb0 = col(2), cb1 = col(3);
+      // Issue order matters: loads complete in order, so the column words
+      // (which have no dependences) go out before the record loads.
+      for (uint32_t k = 0; k < width; k += 4) {
+        const uint32_t na0 = col(k + 4), na1 = col(k + 5);
+        load_nbr<D>(Rr, cb0, i, rb0);
+        load_nbr<D>(Rr, cb1, i, rb1);
+        bond_term<D>(ca0, ra0, Xo, ui, cnot, snot, brk, bc + 64u * k, F);
+        bond_term<D>(ca1, ra1, Xo, ui, cnot, snot, brk, bc + 64u * (k + 1), F);
+        const uint32_t nb0 = col(k + 6), nb1 = col(k + 7);
+        load_nbr<D>(Rr, na0, i, ra0);
+        load_nbr<D>(Rr, na1, i, ra1);
+        bond_term<D>(cb0, rb0, Xo, ui, cnot, snot, brk, bc + 64u * (k + 2), F);
+        bond_term<D>(cb1, rb1, Xo, ui, cnot, snot, brk, bc + 64u * (k + 3), F);
+        ca0 = na0;
+        ca1 = na1;
+        cb0 = nb0;
+        cb1 = nb1;
+      }
+      }
+    }
+
+    // ---- everything after the bond sum
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
-    const NodeRec *Rr = A.rec[t & 1];
     const NodeRec ri = Rr[i];
-    double Xo[D], Xc[D], ui[D], xc[D];
+    double Xc[D], ui[D], xc[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      Xo[k] = ri.X[k];
-      ui[k] = ri.u[k];
-    }
+    for (int k = 0; k < D; ++k) ui[k] = ri.u[k];
     cur_pos<D>(ri, gp, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
@@ -358,9 +470,6 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs A, const uint32_t t
     const bool mov = grain_movable(gp, ip, g, t);
     const double *vprev = A.vel[(t + 1) & 1];
     double *vcur = A.vel[t & 1];
-    double F[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) F[k] = 0.0;
     double vi[D], ai[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -376,28 +485,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs A, const uint32_t t
       for (int q = 0; q < 4; ++q) wt[s][q] = 0.0;
 
     if (mov) {
-      // ---- (a) PMB bond force with bond breaking
       const bool brk = (ip.flags & IF_FRACTURE) && (gp.flags & GF_BREAKABLE);
-      const uint32_t sl = i >> 6, lane = i & 63;
-      const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-      for (uint32_t k = 0; k < width; ++k) {
-        const uint32_t e = base + 64u * k + lane;
-        if (A.bstat[e] != BOND_INTACT) continue;
-        const uint32_t j = A.bcol[e];
-        const NodeRec rj = Rr[j];
-        double xi[D], eta[D];
-#pragma unroll
-        for (int q = 0; q < D; ++q) xi[q] = rj.X[q] - Xo[q];
-        const double xin = sqrt(red<D>(xi, xi));
-#pragma unroll
-        for (int q = 0; q < D; ++q) eta[q] = (rj.u[q] - ui[q]) + xi[q];
-        const double n = sqrt(red<D>(eta, eta));
-        const double s = (n - xin) / xin;
-        const double w = (gp.cnot * s) * rj.X[D];
-#pragma unroll
-        for (int q = 0; q < D; ++q) F[q] = F[q] + w * (eta[q] / n);
-        if (brk && s > gp.snot) A.bstat[e] = BOND_BROKEN;
-      }
 
       // ---- (b) contact from the sorted Verlet list
       const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
@@ -574,7 +662,8 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t, double *gmean) 
   double m[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) m[k] = 0.0;
-  for (uint32_t n = gp.node_begin; n < gp.node_end; ++n) {
+  for (uint32_t e = gp.node_begin; e < gp.node_end; ++e) { // reference node order
+    const uint32_t n = A.ext2int[e];
     double Xc[D];
     cur_pos<D>(Rr[n], gp, Xc);
 #pragma unroll
@@ -811,8 +900,9 @@ __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
     if (A.node_grain[j] == g) {
       for (uint32_t k = 0; k < width; ++k) {
         const uint32_t e = base + 64u * k + lane;
-        if (A.bstat[e] == BOND_PAD) break;
-        if (A.bcol[e] == j) {
+        const uint32_t w = A.bcol[e];
+        if (w == BCOL_PAD) break;
+        if ((w & BCOL_INDEX) == j) {
           slot = (int32_t)e;
           break;
         }

@@ -1,0 +1,447 @@
+"""Pins of the oracle (the FP64 CPU restatement of the reference time loop).
+
+The reference ships no tests, golden vectors or fixtures, and its C++ cannot
+be built here (it needs Eigen 3.3.9).  Each oracle function is therefore
+checked against what the mathematics fixes: closed forms computed here from
+first principles, invariants, special cases and brute force on tiny inputs,
+chosen so that a plausible slip (a sign, the wrong node's volume, a missing
+term, a transposed index, a strict/non-strict comparison) fails a test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from perigrain_amd import setups
+from perigrain_amd.setups import Grain, Params, assemble
+
+DT = 1e-8
+
+
+def grain(pos, vol=None, conn=(), rho=1000.0, cnot=1e20, snot=10.0, delta=1.0, bdry=None):
+    pos = np.atleast_2d(np.asarray(pos, float))
+    n = len(pos)
+    vol = np.full(n, 1e-12) if vol is None else np.asarray(vol, float)
+    conn = np.asarray(conn, np.uint32).reshape(-1, 2)
+    bdry = np.arange(n, dtype=np.uint32) if bdry is None else np.asarray(bdry, np.uint32)
+    return Grain(pos, vol, conn, bdry, delta, rho, cnot, snot, 1.0)
+
+
+def build(grains, dim, Rc=0.0, Kn=0.0, zeta=0.0, mu=0.0, vels=None, wall=None, gravity=0.0,
+          **pkw):
+    vels = [np.zeros(dim)] * len(grains) if vels is None else vels
+    s = assemble(dim, grains, [np.zeros(dim)] * len(grains), vels, Rc, Kn, zeta, mu, gravity,
+                 0 if wall is None else 1, [0.0] * 6 if wall is None else wall)
+    p = Params(dt=DT, **pkw)
+    return s, p
+
+
+def first_force(s, p, steps=1):
+    o = oracle.Oracle(s, p)
+    o.step(steps)
+    return o.read("force"), o
+
+
+# ------------------------------------------------------------- PMB bonds
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_node_bond_force(dim):
+    """F_i = c s V_j e_ij with s = (|eta| - |xi|)/|xi|: the sign (tension
+    pulls i toward j) and the partner's volume (not the node's own)."""
+    L0, L, c = 1e-4, 1.03e-4, 7e19
+    X = np.zeros((2, dim))
+    X[1, 0] = L0
+    g = grain(X, vol=[2e-12, 5e-12], conn=[(0, 1)], cnot=c)
+    s, p = build([g], dim)
+    s.disp[1, 0] = L - L0
+    F, _ = first_force(s, p)
+    st = (L - L0) / L0
+    assert F[0, 0] == pytest.approx(c * st * 5e-12, rel=1e-12)
+    assert F[1, 0] == pytest.approx(-c * st * 2e-12, rel=1e-12)
+    assert np.all(F[:, 1:] == 0)
+
+
+def _lattice(n, h, dim, rng=None, jitter=0.0):
+    ax = np.arange(n) * h
+    pts = np.stack([g.reshape(-1) for g in np.meshgrid(*([ax] * dim), indexing="ij")], axis=1)
+    if rng is not None:
+        pts = pts + rng.uniform(-jitter * h, jitter * h, size=pts.shape)
+    return pts
+
+
+def _bonds(pts, delta):
+    from scipy.spatial import cKDTree
+    pr = cKDTree(pts).query_pairs(delta, output_type="ndarray")
+    return pr[np.lexsort((pr[:, 1], pr[:, 0]))]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_affine_stretch_gives_uniform_stretch_and_breaks_on_threshold(dim):
+    """disp = eps*X stretches every bond by exactly eps (up to rounding);
+    every bond breaks iff eps > s0 (remove_bonds uses a strict '>')."""
+    h = 1e-4
+    pts = _lattice(6 if dim == 2 else 4, h, dim, np.random.default_rng(1), 0.05)
+    conn = _bonds(pts, 3 * h)
+    for eps, s0, expect_broken in ((2e-3, 1e-3, True), (2e-3, 3e-3, False)):
+        g = grain(pts, conn=conn, snot=s0)
+        s, p = build([g], dim, enable_fracture=1)
+        s.disp[:] = eps * s.pos
+        o = oracle.Oracle(s, p)
+        o.step(1)
+        intact = o.bond_intact()
+        assert intact.size == 2 * len(conn)
+        assert (intact == 0).all() if expect_broken else (intact == 1).all()
+    # interior node of a symmetric (unjittered) lattice under uniform stretch
+    pts = _lattice(7, h, dim) if dim == 2 else _lattice(5, h, dim)
+    conn = _bonds(pts, 3 * h)
+    g = grain(pts, conn=conn, snot=1.0)
+    s, p = build([g], dim)
+    s.disp[:] = 1e-3 * s.pos
+    F, _ = first_force(s, p)
+    centre = np.argmin(np.linalg.norm(pts - pts.mean(axis=0), axis=1))
+    scale = np.abs(F).max()
+    assert np.abs(F[centre]).max() <= 1e-9 * scale
+
+
+def test_break_threshold_is_strict():
+    """A bond whose stretch equals s0 survives; one ulp above breaks."""
+    X = np.array([[0.0, 0.0], [1e-4, 0.0]])
+    s, p = build([grain(X, conn=[(0, 1)])], 2, enable_fracture=1)
+    s.disp[1, 0] = 1e-6
+    n = np.sqrt((1e-4 + 1e-6) ** 2 + 0.0)
+    st = (n - 1e-4) / 1e-4
+    for snot, broken in ((st, False), (np.nextafter(st, 0.0), True)):
+        s.snot[:] = snot
+        o = oracle.Oracle(s, p)
+        o.step(1)
+        assert (o.bond_intact() == 0).all() == broken
+
+
+def test_bond_breaking_is_symmetric_and_keeps_row_order():
+    """s_ij == s_ji exactly, so a bond breaks in both rows at once; the
+    surviving Connectivity keeps the adjacency order (NbdArr2conn)."""
+    rng = np.random.default_rng(3)
+    h = 1e-4
+    pts = _lattice(8, h, 2, rng, 0.05)
+    conn = _bonds(pts, 3 * h)
+    g = grain(pts, conn=conn, snot=4e-3)
+    s, p = build([g], 2, enable_fracture=1)
+    s.disp[:] = rng.normal(0, 5e-7, size=s.disp.shape)
+    o = oracle.Oracle(s, p)
+    o.step(1)
+    intact = o.bond_intact()
+    # rebuild the reference adjacency (conn2NArr) to map slots to pairs
+    rows = [[] for _ in range(len(pts))]
+    for a, b in conn:
+        rows[a].append(b)
+        rows[b].append(a)
+    slot = {}
+    k = 0
+    for i, r in enumerate(rows):
+        for j in r:
+            slot[(i, j)] = k
+            k += 1
+    broken = {(i, j) for (i, j), q in slot.items() if intact[q] == 0}
+    assert broken, "test displacement should break some bonds"
+    assert all((j, i) in broken for (i, j) in broken)
+    expect = [(i, j) for i, r in enumerate(rows) for j in r if i < j and (i, j) not in broken]
+    assert np.array_equal(o.conn(0), np.array(expect, np.uint32).reshape(-1, 2))
+
+
+# ----------------------------------------------------------- contact
+
+def _pair(dim, r, vi=None, vj=None, Vj=3e-12, rho=900.0, Rc=2e-5, Kn=5e21, zeta=0.0, mu=0.0,
+          **pkw):
+    e = np.zeros(dim)
+    e[0] = 1.0
+    gi = grain(np.zeros((1, dim)), vol=[1e-12], rho=rho)
+    gj = grain((r * e)[None, :], vol=[Vj], rho=rho)
+    vels = [np.zeros(dim) if vi is None else np.asarray(vi, float),
+            np.zeros(dim) if vj is None else np.asarray(vj, float)]
+    s, p = build([gi, gj], dim, Rc=Rc, Kn=Kn, zeta=zeta, mu=mu, vels=vels, **pkw)
+    return s, p
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_contact_normal_force_closed_form(dim):
+    """Repulsion -K (Rc - r) V_j e_ij on i (away from j), nothing beyond Rc."""
+    r, Rc, K, Vj = 1.5e-5, 2e-5, 5e21, 3e-12
+    s, p = _pair(dim, r, Rc=Rc, Kn=K, Vj=Vj, allow_damping=0, allow_friction=0)
+    F, o = first_force(s, p)
+    assert F[0, 0] == pytest.approx(-K * (Rc - r) * Vj, rel=1e-12)
+    assert F[1, 0] == pytest.approx(+K * (Rc - r) * 1e-12, rel=1e-12)
+    s, p = _pair(dim, 2.5e-5, Rc=Rc, Kn=K, allow_damping=0, allow_friction=0)
+    F, _ = first_force(s, p)
+    assert np.all(F == 0)
+
+
+def _pair_force(xi, xj, vi, vj, Vj, rho, Rc, K, zeta, mu, damping, friction):
+    """Contact force density on i from j, from the physical model: normal
+    spring along e = (x_j - x_i)/r, dashpot on the normal relative velocity,
+    Coulomb friction opposing the tangential relative velocity (with the
+    reference's small-velocity cap tau/(rho dt))."""
+    d = xj - xi
+    r = np.linalg.norm(d)
+    e = d / r
+    fc = -K * (Rc - r) * Vj * e
+    F = fc.copy()
+    w = vi - vj
+    if damping:
+        F += -2 * zeta * math.sqrt(K * rho) * math.sqrt(Vj) * np.dot(w, e) * e
+    if friction:
+        t = w - np.dot(w, e) * e
+        tau = np.linalg.norm(t)
+        if tau > 0:
+            F += (-(tau / rho / DT) if tau < mu * np.linalg.norm(fc) * DT / rho
+                  else -mu * np.linalg.norm(fc)) * t / tau
+    return F
+
+
+def test_contact_damping_closed_form():
+    """Damping -2 zeta sqrt(K rho_i) sqrt(V_j) ((v_i - v_j).e) e on top of
+    the normal force, evaluated where the step-1 position update puts i."""
+    r, Rc, K, Vj, rho, zeta = 1.5e-5, 2e-5, 5e21, 3e-12, 900.0, 0.7
+    w = np.array([2.0, 0.0])
+    s, p = _pair(2, r, vi=w, Rc=Rc, Kn=K, Vj=Vj, rho=rho, zeta=zeta, allow_damping=1,
+                 allow_friction=0)
+    F, o = first_force(s, p)
+    x = o.read("CurrPos")
+    assert x[0, 0] == w[0] * DT
+    want = _pair_force(x[0], x[1], w, np.zeros(2), Vj, rho, Rc, K, zeta, 0.0, True, False)
+    assert F[0] == pytest.approx(want, rel=1e-12, abs=1e-12 * np.abs(want).max())
+    # the damping term alone is a substantial part of the force here
+    assert abs(want[0] + K * (Rc - (r - w[0] * DT)) * Vj) > 1e-3 * abs(want[0])
+
+
+def test_contact_friction_and_small_velocity_cap():
+    """Sliding: -mu |f_n| t_hat; below mu|f_n| dt/rho: -(tau/rho/dt) t_hat."""
+    r, Rc, K, Vj, rho, mu = 1.5e-5, 2e-5, 5e21, 3e-12, 900.0, 0.4
+    for vt, capped in ((50.0, False), (1e-9, True)):
+        s, p = _pair(2, r, vi=[0.0, vt], Rc=Rc, Kn=K, Vj=Vj, rho=rho, mu=mu, allow_damping=0,
+                     allow_friction=1)
+        F, o = first_force(s, p)
+        x = o.read("CurrPos")
+        want = _pair_force(x[0], x[1], np.array([0.0, vt]), np.zeros(2), Vj, rho, Rc, K, 0.0,
+                           mu, False, True)
+        fn = K * (Rc - np.linalg.norm(x[1] - x[0])) * Vj
+        assert (vt < mu * fn * DT / rho) == capped
+        assert F[0] == pytest.approx(want, rel=1e-12, abs=1e-12 * np.abs(want).max())
+        assert F[0, 1] < 0  # friction opposes the sliding direction
+
+
+def test_self_contact_rules():
+    """Same-grain pairs: none while the bond is intact; once broken, contact
+    with R = rho0 and K = c/rho0 when rho0 < Rc (contact.h:686-733)."""
+    d0, Rc, c = 1.0e-5, 2e-5, 3e19
+    X = np.array([[0.0, 0.0], [d0, 0.0]])
+    g = grain(X, conn=[(0, 1)], cnot=c, snot=0.05)
+    s, p = build([g], 2, Rc=Rc, Kn=1.0, enable_fracture=1, self_contact=1, allow_damping=0,
+                 allow_friction=0)
+    s.disp[1, 0] = -0.3 * d0  # compressed: stretch -0.3 < s0, bond intact
+    o = oracle.Oracle(s, p)
+    o.step(1, record_contacts=True)
+    assert len(o.contacts()) == 0
+    # stretch the bond past s0 and squeeze it back within rho0 in one step
+    s.snot[:] = -0.5  # any stretch > -0.5 breaks
+    o = oracle.Oracle(s, p)
+    o.step(1, record_contacts=True)
+    assert (o.bond_intact() == 0).all()
+    pairs = o.contacts()
+    assert pairs.tolist() == [[0, 1], [1, 0]]
+    F = o.read("force")
+    r = 0.7 * d0
+    st = (r - d0) / d0
+    bond = c * st * 1e-12  # the bond still acts in the step it breaks (along +x)
+    contact = -(c / d0) * (d0 - r) * 1e-12  # K = c/rho0, R = rho0, pushes i away
+    assert F[0, 0] == pytest.approx(bond + contact, rel=1e-12)
+    assert F[1, 0] == pytest.approx(-(bond + contact), rel=1e-12)
+
+
+def test_bbox_pruned_pairs_equal_brute_force():
+    """get_minmax + intersects_box_fext prune grain pairs without losing any
+    node pair within Rc: the contact list equals an O(N^2) search."""
+    rng = np.random.default_rng(5)
+    h, Rc = 1e-4, 6e-5
+    grains = []
+    for k in range(6):
+        pts = _lattice(4, h, 2, rng, 0.05) + rng.uniform(0, 7e-4, size=2)
+        grains.append(grain(pts, conn=_bonds(pts, 1.5 * h)))
+    s, p = build(grains, 2, Rc=Rc, Kn=1e10, allow_damping=0, allow_friction=0)
+    o = oracle.Oracle(s, p)
+    o.step(1, record_contacts=True)
+    got = {tuple(x) for x in o.contacts().tolist()}
+    x = o.read("CurrPos")
+    gid = np.repeat(np.arange(s.n_grains), np.diff(s.node_offset))
+    want = set()
+    for i in range(len(x)):
+        for j in range(len(x)):
+            if gid[i] != gid[j] and np.sqrt(np.sum((x[j] - x[i]) ** 2)) <= Rc:
+                want.add((i, j))
+    assert want and got == want
+
+
+# ---------------------------------------------------------------- walls
+
+def test_segment_mass_2d_is_circular_segment_area():
+    """2D area = R^2 acos(d/R) - d sqrt(R^2-d^2) (numerical integration);
+    centroid 0 because the reference computes 2/3 in integer arithmetic."""
+    R = 2e-5
+    for d in (0.0, 0.3 * R, 0.9 * R):
+        c, a = oracle.segment_mass(2, R, d)
+        y = np.linspace(d, R, 200001)
+        num = np.trapezoid(2 * np.sqrt(np.maximum(R * R - y * y, 0)), y)
+        assert a == pytest.approx(num, rel=1e-6)
+        assert c == 0.0
+
+
+def test_segment_mass_3d_special_cases():
+    """3D 'area' pi/3 (R-d)^2 (2R-d): the hemisphere 2 pi R^3/3 at d=0,
+    zero at d=R (the reference's formula; the true cap volume would have
+    2R+d); centroid I/A with I = pi/4 (R^2-d^2)^2."""
+    R = 2e-5
+    c, a = oracle.segment_mass(3, R, 0.0)
+    assert a == pytest.approx(2 * math.pi * R ** 3 / 3, rel=1e-15)
+    assert c == pytest.approx((math.pi / 4 * R ** 4) / a, rel=1e-15)
+    _, a = oracle.segment_mass(3, R, R)
+    assert a == 0.0
+    _, a_half = oracle.segment_mass(3, R, 0.5 * R)
+    assert a_half == pytest.approx(math.pi * R ** 3 / 8, rel=1e-15)
+
+
+def test_wall_normal_force_and_reaction_2d():
+    """Node at distance d above the bottom wall: F_y = K_n Rc A(d) (centroid
+    0), reaction on the wall = -V F."""
+    Rc, K, V, d = 2e-5, 1e16, 2e-12, 0.4 * 2e-5
+    g = grain(np.array([[0.0, d]]), vol=[V])
+    s, p = build([g], 2, Rc=Rc, Kn=K, wall=[-1.0, 1.0, 1.0, 0.0], allow_damping=0,
+                 allow_friction=0)
+    F, o = first_force(s, p)
+    A = Rc * Rc * math.acos(d / Rc) - d * math.sqrt(Rc * Rc - d * d)
+    assert F[0, 1] == pytest.approx(K * Rc * A, rel=1e-12)
+    _, react = o.wall()
+    assert react[3, 1] == pytest.approx(-V * F[0, 1], rel=1e-12)  # bottom
+    assert np.all(react[:3] == 0)
+
+
+def test_wall_moving_relative_velocity_3d():
+    """Damping uses the velocity relative to the moving wall."""
+    Rc, K, V, d, zeta, rho = 2e-5, 1e16, 2e-12, 0.5e-5, 0.5, 1000.0
+    g = grain(np.array([[0.0, 0.0, 1.0 - d]]), vol=[V], rho=rho)
+    wall = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    s, p = build([g], 3, Rc=Rc, Kn=K, zeta=zeta, wall=wall, allow_damping=1, allow_friction=0)
+    p.wall_speed[5] = -3.0
+    F, o = first_force(s, p)
+    dist = abs((1.0 - d) - 1.0)  # the signed distance as computed in floating point
+    c, A = oracle.segment_mass(3, Rc, dist)
+    assert c < Rc and A > 0
+    fn = -(K * max(Rc - c, 0.0) * A)  # node below the top wall: pushed down
+    fd = -(2 * zeta * math.sqrt(K * rho) * math.sqrt(abs(A))) * (0.0 - (-3.0))
+    assert F[0, 2] == pytest.approx(fn + fd, rel=1e-12)
+    w, _ = o.wall()
+    assert w[5] == 1.0 + (-3.0) * DT
+
+
+# --------------------------------------------------------------- Verlet
+
+def test_free_flight_is_exact_parabola():
+    """Constant force density: x(t) = x0 + v0 t + a t^2/2 to rounding."""
+    g = grain(np.array([[0.0, 0.0]]), rho=500.0)
+    s, p = build([g], 2, vels=[np.array([3.0, -1.0])])
+    s.extforce[:] = [500.0 * 2.0, 500.0 * -7.0]
+    o = oracle.Oracle(s, p)
+    n = 1000
+    o.step(n)
+    t = n * DT
+    x = o.read("CurrPos")[0]
+    assert x[0] == pytest.approx(3.0 * t + 0.5 * 2.0 * t * t, rel=1e-10)
+    assert x[1] == pytest.approx(-1.0 * t + 0.5 * -7.0 * t * t, rel=1e-10)
+    assert np.allclose(o.read("acc")[0], [2.0, -7.0], rtol=1e-15)
+
+
+def test_two_node_oscillator_energy_is_bounded():
+    """A bond is a linear spring for small stretch: velocity Verlet keeps the
+    energy within a small band over many periods."""
+    L0, c, V, rho = 1e-4, 1e20, 1e-12, 1000.0
+    X = np.array([[0.0, 0.0], [L0, 0.0]])
+    g = grain(X, vol=[V, V], conn=[(0, 1)], cnot=c, rho=rho)
+    s, p = build([g], 2)
+    s.disp[1, 0] = 1e-3 * L0
+    o = oracle.Oracle(s, p)
+    k = c * V * V / L0  # force = k * extension (force density * volume)
+    E = []
+    for _ in range(60):
+        o.step(50)
+        x = o.read("CurrPos")
+        v = o.read("vel")
+        ext = (x[1, 0] - x[0, 0]) - L0
+        E.append(0.5 * rho * V * np.sum(v ** 2) + 0.5 * k * ext ** 2)
+    E = np.array(E)
+    assert (E.max() - E.min()) / E.mean() < 0.05
+
+
+def test_clamped_nodes_stay_fixed():
+    X = _lattice(4, 1e-4, 2)
+    g = grain(X, conn=_bonds(X, 1.5e-4))
+    s, p = build([g], 2, vels=[np.array([1.0, 2.0])])
+    s.clamped_offset = np.array([0, 2], np.int64)
+    s.clamped_nodes = np.array([0, 5], np.uint32)
+    o = oracle.Oracle(s, p)
+    o.step(20)
+    for f in ("vel", "acc", "disp"):
+        assert np.all(o.read(f)[[0, 5]] == 0)
+    assert np.all(o.read("CurrPos")[[0, 5]] == X[[0, 5]])
+
+
+def test_gradient_extforce_ramp():
+    g = grain(np.array([[0.0, 0.0]]))
+    s, p = build([g], 2, gradient_extforce=1, extforce_maxstep=40)
+    s.extforce[:] = [4.0, 8.0]
+    o = oracle.Oracle(s, p)
+    o.step(10)
+    assert np.allclose(o.read("force")[0], [1.0, 2.0], rtol=1e-15)
+    o.step(60)
+    assert np.allclose(o.read("force")[0], [4.0, 8.0], rtol=1e-15)
+
+
+def test_set_movable_toggle():
+    """A grain made movable at step T stays put before T."""
+    g0 = grain(np.array([[0.0, 0.0]]))
+    s, p = build([g0], 2, vels=[np.array([1.0, 0.0])], set_movable_index=0,
+                 set_movable_timestep=5)
+    s.movable[:] = 0
+    o = oracle.Oracle(s, p)
+    o.step(4)
+    assert o.read("CurrPos")[0, 0] == 0.0
+    assert np.all(o.read("force") == 0)
+    o.step(2)
+    assert o.read("CurrPos")[0, 0] > 0.0
+
+
+def test_torque_is_tangential():
+    """Torque density tau on a ring: each node's force is tau along the
+    counter-clockwise tangent."""
+    ang = np.linspace(0, 2 * np.pi, 8, endpoint=False)
+    X = 1e-4 * np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    g = grain(X)
+    s, p = build([g], 2, enable_torque=1)
+    s.torque_val[:] = 5.0
+    F, _ = first_force(s, p)
+    tang = np.stack([-np.sin(ang), np.cos(ang)], axis=1)
+    assert np.allclose(F, 5.0 * tang, atol=1e-12)
+
+
+# ------------------------------------------------------------ generators
+
+def test_generated_bonds_are_within_horizon_and_unique():
+    s, p = setups.two_disk_collision(seed=2)
+    for gi in range(s.n_grains):
+        x = s.pos[s.grain_nodes(gi)]
+        c = s.conn[s.conn_offset[gi]:s.conn_offset[gi + 1]].astype(int)
+        d = np.linalg.norm(x[c[:, 1]] - x[c[:, 0]], axis=1)
+        assert np.all(d <= s.delta[gi])
+        assert np.all(c[:, 0] < c[:, 1])
+        assert len({tuple(r) for r in c.tolist()}) == len(c)
+        # every pair within delta is bonded (convex disk)
+        from scipy.spatial import cKDTree
+        assert len(cKDTree(x).query_pairs(s.delta[gi])) == len(c)
