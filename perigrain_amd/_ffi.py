@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPG_PATH = os.path.join(_HERE, "libpg.so")
+# PERIGRAIN_LIBPG selects another build of the library (e.g. a tuning variant)
+LIBPG_PATH = os.environ.get("PERIGRAIN_LIBPG") or os.path.join(_HERE, "libpg.so")
 
 PG_OK = 0
 PG_MAX_SIDES = 6

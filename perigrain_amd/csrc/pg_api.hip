@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +41,12 @@ int fail(int code, const std::string &msg) {
       return fail(PG_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Host staging of one node (reference position, volume, displacement).
+struct HostRec {
+  double X[4];
+  double u[4];
+};
+
 struct InstHost {
   uint32_t node_begin, node_end, grain_begin, grain_end;
   uint64_t slot_begin, slot_end;
@@ -65,10 +72,14 @@ struct pg_handle {
   std::vector<uint32_t> deg, slice_base, slice_width, bcol, grain_node_begin;
   std::vector<uint64_t> slot0;
   std::vector<uint32_t> perm, inv; // device index <-> reference index
-  bool reorder = true;
+  // nodes are degree-sorted within windows of this many nodes of a grain
+  // (0: reference order, which measured fastest: it keeps the gathers local)
+  uint32_t sort_window = 0;
+  bool uniform_vol = false;
   std::vector<void *> allocs;
   // device
-  NodeRec *rec[2] = {nullptr, nullptr};
+  uint32_t ns = 0; // node stride of the component-major arrays
+  double *rec[2] = {nullptr, nullptr}, *V = nullptr; // [ns][2*dim] records, [ns] volumes
   double *vel[2] = {nullptr, nullptr};
   double *acc = nullptr, *ext = nullptr, *xb = nullptr, *gmean = nullptr;
   uint32_t *node_grain = nullptr, *ckey = nullptr, *ext2int = nullptr;
@@ -144,6 +155,8 @@ void fill_args(pg_handle *h) {
   StepArgs &A = h->A;
   A.rec[0] = h->rec[0];
   A.rec[1] = h->rec[1];
+  A.V = h->V;
+  A.ns = h->ns;
   A.vel[0] = h->vel[0];
   A.vel[1] = h->vel[1];
   A.acc = h->acc;
@@ -173,6 +186,7 @@ void fill_args(pg_handle *h) {
   A.n_grains = h->G;
   A.n_insts = h->I;
   A.any_torque = h->any_torque ? 1 : 0;
+  A.uniform_vol = h->uniform_vol ? 1 : 0;
 }
 
 int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
@@ -196,7 +210,7 @@ int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
 }
 
 // Build the contact Verlet list from the positions of step ts
-// (u^ts in rec[ts & 1]).
+// (u^ts in u[ts & 1]).
 int build_list(pg_handle *h, uint32_t ts) {
   const int D = h->dim;
   if (!h->any_contact) {
@@ -335,6 +349,11 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   for (int s = 0; s < n; ++s) RC(check_setup(&setups[s], &params[s], D));
   h->dim = D;
   h->device = device;
+  {
+    // PG_SORT_WINDOW=n overrides the degree-sort window (0: reference order)
+    const char *w = std::getenv("PG_SORT_WINDOW");
+    if (w && *w) h->sort_window = (uint32_t)std::strtoul(w, nullptr, 10);
+  }
 
   // ---- sizes and per-instance ranges (host-side validation and staging
   // come first, so a bad setup is reported before any device work)
@@ -360,7 +379,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   h->I = (uint32_t)n;
 
   // ---- host staging of node data
-  std::vector<NodeRec> rec(N);
+  std::vector<HostRec> rec(N);
   std::vector<double> vel(N * D), acc(N * D), ext(N * D);
   std::vector<uint32_t> node_grain(N), ckey(N, 0xFFFFFFFFu);
   std::vector<uint8_t> node_info(N, 0);
@@ -382,7 +401,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
       h->grain_node_begin[gg] = ih.node_begin + (uint32_t)a;
       for (int64_t q = a; q < b; ++q) {
         const uint64_t i = ih.node_begin + q;
-        NodeRec &r = rec[i];
+        HostRec &r = rec[i];
         std::memset(&r, 0, sizeof(r));
         for (int k = 0; k < D; ++k) {
           r.X[k] = st.pos[q * D + k];
@@ -481,6 +500,21 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   }
   h->grain_node_begin[G] = (uint32_t)N;
   h->CK = (uint32_t)CK;
+  // grains whose nodes all carry bit-identical volumes (lattice grains) let
+  // the bond loop take the partner volume from the grain instead of a gather
+  h->uniform_vol = true;
+  for (uint32_t gg = 0; gg < G; ++gg) {
+    const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
+    bool same = true;
+    for (uint32_t q = a + 1; q < b && same; ++q)
+      same = std::memcmp(&rec[q].X[D], &rec[a].X[D], sizeof(double)) == 0;
+    if (same) {
+      grains[gg].flags |= GF_UNIFORM_VOL;
+      grains[gg].vol = rec[a].X[D];
+    } else {
+      h->uniform_vol = false;
+    }
+  }
 
   // ---- adjacency rows in the reference's NbdArr order
   for (uint64_t i = 0; i < N; ++i) row_ptr[i + 1] = row_ptr[i] + h->deg[i];
@@ -516,8 +550,9 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     h->inst[s].slot_end = row_ptr[h->inst[s].node_end];
   }
   // ---- device node order: within each grain, nodes sorted by degree
-  // (descending, stable), so the 64 rows of a slice have nearly equal
-  // lengths.  perm maps device index -> reference index, inv the reverse.
+  // (descending, stable) inside windows of sort_window nodes, so the 64 rows
+  // of a slice have nearly equal lengths while neighbouring nodes stay close
+  // in memory.  perm maps device index -> reference index, inv the reverse.
   // Everything order-sensitive (row order, contact order keys, the torque
   // mean) keeps the reference order.
   std::vector<uint32_t> &perm = h->perm, &inv = h->inv;
@@ -526,9 +561,10 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   for (uint32_t gg = 0; gg < G; ++gg) {
     const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
     for (uint32_t q = a; q < b; ++q) perm[q] = q;
-    if (h->reorder)
-      std::stable_sort(perm.begin() + a, perm.begin() + b,
-                       [&](uint32_t x, uint32_t y) { return h->deg[x] > h->deg[y]; });
+    if (h->sort_window > 1)
+      for (uint32_t w0 = a; w0 < b; w0 += h->sort_window)
+        std::stable_sort(perm.begin() + w0, perm.begin() + std::min(b, w0 + h->sort_window),
+                         [&](uint32_t x, uint32_t y) { return h->deg[x] > h->deg[y]; });
   }
   for (uint64_t i = 0; i < N; ++i) inv[perm[i]] = (uint32_t)i;
   auto permute = [&](auto &v, size_t w) {
@@ -555,6 +591,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     uint32_t w = 0;
     for (uint64_t i = (uint64_t)sl * SLICE; i < std::min<uint64_t>(N, (uint64_t)(sl + 1) * SLICE); ++i)
       w = std::max(w, h->deg[perm[i]]);
+    w = (w + BGROUP - 1) / BGROUP * BGROUP; // whole slot groups
     h->slice_base[sl] = (uint32_t)E;
     h->slice_width[sl] = w;
     E += (uint64_t)w * SLICE;
@@ -562,12 +599,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   }
   h->E = E;
   h->bcol.assign(E, 0);
-  std::vector<uint32_t> bcol_init(E + (uint64_t)SLICE * BCOL_TAIL, BCOL_PAD);
+  std::vector<uint32_t> bcol_init(E + BCOL_TAIL, BCOL_PAD);
   for (uint64_t i = 0; i < N; ++i) {
     const uint32_t sl = (uint32_t)(i / SLICE), lane = (uint32_t)(i % SLICE);
     const uint32_t r = perm[i];
     for (uint32_t k = 0; k < h->deg[r]; ++k) {
-      const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
+      const uint64_t e = bslot(h->slice_base[sl], lane, k);
       h->bcol[e] = inv[rows[row_ptr[r] + k]];
       bcol_init[e] = h->bcol[e];
     }
@@ -577,9 +614,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
-  const size_t ND = (size_t)N * D;
-  RC(dalloc(h, h->rec[0], N));
-  RC(dalloc(h, h->rec[1], N));
+  h->ns = (uint32_t)(((N + SLICE - 1) / SLICE) * SLICE);
+  if (h->ns == 0) h->ns = SLICE;
+  const size_t ND = (size_t)h->ns * D; // component-major [D][ns]
+  RC(dalloc(h, h->rec[0], 2 * ND));
+  RC(dalloc(h, h->rec[1], 2 * ND));
+  RC(dalloc(h, h->V, h->ns));
   RC(dalloc(h, h->vel[0], ND));
   RC(dalloc(h, h->vel[1], ND));
   RC(dalloc(h, h->acc, ND));
@@ -594,7 +634,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->insts, n));
   RC(dalloc(h, h->d_slice_base, S));
   RC(dalloc(h, h->d_slice_width, S));
-  RC(dalloc(h, h->d_bcol, E + (size_t)SLICE * BCOL_TAIL));
+  RC(dalloc(h, h->d_bcol, E + BCOL_TAIL));
   RC(dalloc(h, h->cl_ptr, (size_t)N + 1));
   RC(dalloc(h, h->cl_cnt, (size_t)N + 1));
   h->cl_cap = 1024;
@@ -618,13 +658,30 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     h->h_pin = static_cast<uint32_t *>(p);
   }
 
-  // ---- uploads
-  RC(h2d(h, h->rec[0], rec.data(), N));
-  RC(h2d(h, h->rec[1], rec.data(), N));
-  RC(h2d(h, h->vel[0], vel.data(), ND));
-  RC(h2d(h, h->vel[1], vel.data(), ND));
-  RC(h2d(h, h->acc, acc.data(), ND));
-  RC(h2d(h, h->ext, ext.data(), ND));
+  // ---- uploads (records node-major, the rest transposed to component-major)
+  {
+    const uint32_t nsd = h->ns;
+    std::vector<double> sR(2 * ND, 0.0), sV(nsd, 0.0), sVel(ND, 0.0), sAcc(ND, 0.0),
+        sExt(ND, 0.0);
+    for (uint64_t i = 0; i < N; ++i) {
+      sV[i] = rec[i].X[D];
+      for (int k = 0; k < D; ++k) {
+        sR[i * 2 * D + k] = rec[i].X[k];
+        sR[i * 2 * D + D + k] = rec[i].u[k];
+        sVel[(size_t)k * nsd + i] = vel[i * D + k];
+        sAcc[(size_t)k * nsd + i] = acc[i * D + k];
+        sExt[(size_t)k * nsd + i] = ext[i * D + k];
+      }
+    }
+    RC(h2d(h, h->rec[0], sR.data(), 2 * ND));
+    RC(h2d(h, h->rec[1], sR.data(), 2 * ND));
+    RC(h2d(h, h->V, sV.data(), (size_t)nsd));
+    RC(h2d(h, h->vel[0], sVel.data(), ND));
+    RC(h2d(h, h->vel[1], sVel.data(), ND));
+    RC(h2d(h, h->acc, sAcc.data(), ND));
+    RC(h2d(h, h->ext, sExt.data(), ND));
+    HIPCHK(hipStreamSynchronize(h->stream)); // staging buffers go out of scope
+  }
   RC(h2d(h, h->node_grain, node_grain.data(), N));
   RC(h2d(h, h->node_info, node_info.data(), N));
   RC(h2d(h, h->ckey, ckey.data(), N));
@@ -781,12 +838,19 @@ int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
   HIPCHK(hipSetDevice(h->device));
   // device order -> reference order (the permutation stays inside the range)
   auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
+  // component k of the instance's node range of a [D][ns] array
+  auto comp = [&](const double *base, int k, std::vector<double> &v) -> int {
+    v.resize(n);
+    return d2h(h, v.data(), base + (size_t)k * h->ns + ih.node_begin, n);
+  };
   if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
-    std::vector<NodeRec> r(n);
-    RC(d2h(h, r.data(), h->rec[h->t & 1] + ih.node_begin, n));
+    // records [node][2*D]: X then u; CurrPos = pos + disp
+    std::vector<double> r(n * 2 * D);
+    RC(d2h(h, r.data(), h->rec[h->t & 1] + (size_t)ih.node_begin * 2 * D, r.size()));
     for (size_t e = 0; e < n; ++e) {
-      const NodeRec &q = r[at(e)];
-      for (int k = 0; k < D; ++k) dst[e * D + k] = field == PG_FIELD_DISP ? q.u[k] : q.X[k] + q.u[k];
+      const double *q = &r[at(e) * 2 * D];
+      for (int k = 0; k < D; ++k)
+        dst[e * D + k] = field == PG_FIELD_CURRPOS ? q[k] + q[D + k] : q[D + k];
     }
     return PG_OK;
   }
@@ -797,10 +861,11 @@ int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
     case PG_FIELD_FORCE: src = h->force_out; break;
     default: return fail(PG_ERR_INVALID, "unknown field");
   }
-  std::vector<double> v(n * D);
-  RC(d2h(h, v.data(), src + (size_t)ih.node_begin * D, n * D));
-  for (size_t e = 0; e < n; ++e)
-    for (int k = 0; k < D; ++k) dst[e * D + k] = v[at(e) * D + k];
+  std::vector<double> v;
+  for (int k = 0; k < D; ++k) {
+    RC(comp(src, k, v));
+    for (size_t e = 0; e < n; ++e) dst[e * D + k] = v[at(e)];
+  }
   return PG_OK;
 }
 
@@ -827,7 +892,7 @@ int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst) {
     const uint32_t sl = i / SLICE, lane = i % SLICE;
     for (uint32_t k = 0; k < h->deg[e]; ++k)
       dst[h->slot0[e] - ih.slot_begin + k] =
-          (st[h->slice_base[sl] + (uint64_t)SLICE * k + lane] & BCOL_BROKEN) ? 0 : 1;
+          (st[bslot(h->slice_base[sl], lane, k)] & BCOL_BROKEN) ? 0 : 1;
   }
   return PG_OK;
 }
@@ -848,7 +913,7 @@ int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_
     const uint32_t i = h->inv[ge];
     const uint32_t sl = i / SLICE, lane = i % SLICE;
     for (uint32_t k = 0; k < h->deg[ge]; ++k) {
-      const uint64_t e = h->slice_base[sl] + (uint64_t)SLICE * k + lane;
+      const uint64_t e = bslot(h->slice_base[sl], lane, k);
       if (st[e] & BCOL_BROKEN) continue;
       const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
       if (li < q) {

@@ -2,20 +2,25 @@
 // gfx950 kernels.
 //
 // Layout in HBM (one handle may pack several independent instances):
-//  * nodes are packed instance by instance, grain by grain, in the order of
-//    the reference's P_%05u groups, so node n of grain g sits at
-//    grain.node_begin + n;
-//  * NodeRec (64 B, two copies for the two step parities): the reference
-//    position X (pos), nodal volume V and the displacement u (disp) in one
-//    aligned half cache line, because the bond and contact gathers need all
-//    three for each partner;
-//  * vel is double-buffered (partners read the previous step's velocity
-//    while their own is being updated); acc and extforce are single arrays;
-//  * bonds are stored as sliced ELL with slices of 64 rows (one wavefront):
-//    entry k of row i lives at slice_base[i/64] + 64*k + i%64, so a
-//    wavefront's loads of bond k are contiguous.  Row order is the
-//    reference's NbdArr order (conn2NArr, load_hdf5.h:28-38), which fixes the
-//    summation order of the bond force;
+//  * nodes are packed instance by instance and grain by grain; inside a
+//    grain they are stored in degree order within small windows (see
+//    pg_api.hip), and perm/inv map between that device order and the
+//    reference's node order;
+//  * what a bond gathers from its partner -- reference position X and
+//    displacement u -- sits in one record of 2*dim doubles (48 B in 3D, 32 B
+//    in 2D), loaded with 16-byte loads (3 or 2 per bond); the partner volume
+//    comes from the grain when all its nodes share one volume (lattice
+//    grains), else from V.  Records are double-buffered by step parity
+//    (X is stored in both): partners read u of step t while a node writes
+//    u of step t+1;
+//  * the other node data are structure-of-arrays, component-major with a
+//    padded stride `ns` (vel[k*ns + i]), vel double-buffered like u;
+//  * bonds are sliced ELL with slices of 64 rows (one wavefront) and rows
+//    padded to a multiple of 4 slots; slot k of row i lives at
+//    slice_base[i/64] + 256*(k/4) + 4*(i%64) + k%4, so one 16-byte load gives
+//    a lane four consecutive slots and a wavefront's load is contiguous.  Row
+//    order is the reference's NbdArr order (conn2NArr, load_hdf5.h:28-38),
+//    which fixes the bond-force summation order;
 //  * contact candidates are a Verlet list built with a skin and kept sorted
 //    per node in the order the reference accumulates contact forces.
 #ifndef PG_INTERNAL_H
@@ -31,14 +36,21 @@ constexpr uint32_t BCOL_BROKEN = 0x80000000u;
 constexpr uint32_t BCOL_INDEX = 0x7FFFFFFFu;
 constexpr uint32_t BCOL_PAD = 0xFFFFFFFFu;
 
-constexpr int SLICE = 64; // rows per sliced-ELL slice (= wavefront width)
-constexpr int BCOL_TAIL = 8; // padding slots past the last slice (bond-loop read-ahead)
+constexpr int SLICE = 64;     // rows per sliced-ELL slice (= wavefront width)
+constexpr int BGROUP = 4;     // slots per lane per column load
+constexpr int BCOL_TAIL = 512; // padding slots past the last slice (bond-loop read-ahead)
+
+// Slot k of the row in lane `lane` of a slice starting at `base`.
+__host__ __device__ inline uint64_t bslot(uint64_t base, uint32_t lane, uint32_t k) {
+  return base + (uint64_t)(k >> 2) * (SLICE * BGROUP) + lane * BGROUP + (k & 3);
+}
 
 // grain flag bits
 constexpr int32_t GF_MOVABLE = 1;
 constexpr int32_t GF_BREAKABLE = 2;
 constexpr int32_t GF_STOPPABLE = 4;
-constexpr int32_t GF_YSHIFT = 8; // pos(1) carries a reset_partzero_y shift
+constexpr int32_t GF_YSHIFT = 8;       // pos(1) carries a reset_partzero_y shift
+constexpr int32_t GF_UNIFORM_VOL = 16; // every node of the grain has volume GrainDev::vol
 
 // instance flag bits
 constexpr int32_t IF_FRACTURE = 1;
@@ -54,11 +66,6 @@ constexpr int32_t IF_TORQUE = 128;
 constexpr uint8_t NI_CLAMPED = 1;
 constexpr uint8_t NI_CONTACT = 2; // takes part in node-node and wall contact
 
-struct alignas(64) NodeRec {
-  double X[4]; // pos[0..dim-1]; X[dim] = vol (3D: X[3]; 2D: X[2], X[3] = 0)
-  double u[4]; // disp[0..dim-1]
-};
-
 struct GrainDev {
   double rho, cnot, snot, yshift;
   int32_t flags;
@@ -67,6 +74,7 @@ struct GrainDev {
   uint32_t torque_axis;
   int32_t pad;
   double torque_val;
+  double vol; // the common nodal volume when GF_UNIFORM_VOL is set
 };
 
 struct InstDev {
@@ -75,7 +83,7 @@ struct InstDev {
   double speed[6];
   int32_t flags;
   uint32_t extforce_maxstep;
-  int32_t set_movable_grain;  // packed grain index or -1
+  int32_t set_movable_grain; // packed grain index or -1
   uint32_t set_movable_step;
   int32_t set_stoppable_grain;
   uint32_t set_stoppable_step;
@@ -96,12 +104,14 @@ struct GridDev {
   double r2; // squared Verlet-list radius (contact radius + skin, with margin)
 };
 
-// Everything the per-step kernel touches (passed by value as kernarg).
+// Everything the kernels touch (passed by value as kernarg).  Records are
+// [node][2*dim] (X then u); the other node arrays are [dim][ns].
 struct StepArgs {
-  NodeRec *rec[2];
-  double *vel[2];
-  double *acc;
-  const double *ext;
+  double *rec[2];     // [n][2*dim] pos and disp, by step parity
+  const double *V;    // [ns] nodal volumes
+  double *vel[2];     // [dim][ns] velocity, by step parity
+  double *acc;        // [dim][ns]
+  const double *ext;  // [dim][ns] extforce (force density)
   const uint32_t *node_grain;
   const uint8_t *node_info;
   const uint32_t *ckey;
@@ -114,18 +124,20 @@ struct StepArgs {
   const uint32_t *cl_ptr;
   const uint32_t *cl_j;
   const int32_t *cl_slot;
-  const double *xb;      // positions at the last list build
+  const double *xb;            // [dim][ns] positions at the last list build
   unsigned long long *bbox[3]; // ordered keys, [grains][2*dim] (min..., max...)
-  double *wallpos[2];    // [insts][6]
-  const double *gmean;   // [grains][3] mean CurrPos (torque)
-  uint32_t *violation;   // first step whose positions broke the skin
-  double *force_out;     // record steps
-  double *wterm;         // record steps: [ckeys][6][4]
-  uint8_t *wflag;        // record steps: [ckeys]
+  double *wallpos[2];          // [insts][6]
+  const double *gmean;         // [grains][3] mean CurrPos (torque)
+  uint32_t *violation;         // first step whose positions broke the skin
+  double *force_out;           // [dim][ns], record steps
+  double *wterm;               // record steps: [ckeys][6][4]
+  uint8_t *wflag;              // record steps: [ckeys]
+  uint32_t ns;                 // node stride (n_nodes rounded up)
   uint32_t n_nodes;
   uint32_t n_grains;
   uint32_t n_insts;
   uint32_t any_torque;
+  uint32_t uniform_vol; // every grain has GF_UNIFORM_VOL
 };
 
 // Order-preserving map of an IEEE double onto uint64 (for atomic min/max).

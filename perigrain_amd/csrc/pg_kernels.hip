@@ -3,10 +3,11 @@
 // Arithmetic contract: every floating-point expression below is written in
 // the operation order of the reference (file:line cited per function) and
 // the file is compiled with -ffp-contract=off, so each +,-,*,/ is a single
-// IEEE binary64 rounding and no FMA is formed.  sqrt and / are the
-// correctly rounded HIP forms.  3-component norms and dot products sum as
-// (a0*b0 + a1*b1) + a2*b2, the order of Eigen 3.3.9's vectorised redux that
-// the reference's norm()/dot() use.
+// IEEE binary64 rounding and no FMA is formed.  Division and square root are
+// correctly rounded (the IEEE operators, or the shortened sequences below
+// where they are provably identical).  3-component norms and dot products
+// sum as (a0*b0 + a1*b1) + a2*b2, the order of Eigen 3.3.9's vectorised
+// redux that the reference's norm()/dot() use.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,10 @@
 #ifndef PG_STEP_WAVES_PER_EU
 #define PG_STEP_WAVES_PER_EU 4
 #endif
+// 1: bond loop uses the shortened division/square-root sequences
+#ifndef PG_FAST_DIVSQRT
+#define PG_FAST_DIVSQRT 1
+#endif
 
 namespace pg {
 
@@ -31,6 +36,67 @@ template <> __device__ __forceinline__ double red<2>(const double *a, const doub
 }
 template <> __device__ __forceinline__ double red<3>(const double *a, const double *b) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// ---- correctly rounded division and square root without range fix-ups.
+//
+// The compiler lowers an IEEE binary64 a/b on gfx950 to
+//   v_div_scale (den), v_rcp, two Newton steps r += r*(1-b*r), v_div_scale
+//   (num), q = a*r, rem = a - b*q (fma), v_div_fmas q + rem*r, v_div_fixup,
+// and sqrt(x) to a scale-by-2^256 test, v_rsq, a Goldschmidt/Newton pair of
+// corrections and a special-value select.  For operands well inside the
+// normal range the scale steps leave the operands unchanged and the fix-ups
+// pass the result through, so the sequences below -- the same operations
+// minus those steps -- return bit-identical, correctly rounded results
+// there.  The reciprocal depends on the denominator only, so one serves
+// several numerators.  Callers check the range (fast_*_ok) and otherwise use
+// the IEEE operators.
+__device__ __forceinline__ double recip_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  return r;
+}
+__device__ __forceinline__ double div_nr(double a, double b, double r) {
+  const double q = a * r;
+  const double rem = fma(-b, q, a);
+  return fma(rem, r, q);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r;
+  double h = r * 0.5;
+  const double e = fma(-h, g, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return g;
+}
+// Ranges where the shortened sequences equal the IEEE ones: denominator in
+// [2^-300, 2^300]; numerator zero (handled by div_r) or in
+// [2^-300, 2^300]; square-root argument in [2^-600, 2^600].
+__device__ __forceinline__ bool fast_den_ok(double b) { return b >= 0x1p-300 && b <= 0x1p300; }
+__device__ __forceinline__ bool fast_num_ok(double a) {
+  const double m = fabs(a);
+  return a == 0.0 || (m >= 0x1p-300 && m <= 0x1p300);
+}
+__device__ __forceinline__ bool fast_sqrt_ok(double x) { return x >= 0x1p-600 && x <= 0x1p600; }
+// a/b through a precomputed reciprocal; a zero numerator returns itself
+// (the IEEE quotient of +-0 by a positive number).
+__device__ __forceinline__ double div_r(double a, double b, double r) {
+  return a == 0.0 ? a : div_nr(a, b, r);
+}
+// Stand-alone forms (range-checked), used by the diagnostics kernel.
+__device__ __forceinline__ double div_rn(double a, double b) {
+  return (fast_den_ok(b) && fast_num_ok(a)) ? div_r(a, b, recip_nr(b)) : a / b;
+}
+__device__ __forceinline__ double sqrt_rn(double x) {
+  return fast_sqrt_ok(x) ? sqrt_nr(x) : sqrt(x);
 }
 
 // Logical block index that gives each XCD (blocks are dealt round-robin to
@@ -53,12 +119,42 @@ __device__ __forceinline__ bool grain_stoppable(const GrainDev &gp, const InstDe
          (ip.set_stoppable_grain == (int32_t)g && t >= ip.set_stoppable_step);
 }
 
+// Record accessors: records are [node][2*D], X then u.
+template <int D>
+__device__ __forceinline__ void rec_X(const StepArgs &A, uint32_t i, double *X) {
+  const double *r = A.rec[0] + (size_t)i * (2 * D); // X is stored in both parities
+#pragma unroll
+  for (int k = 0; k < D; ++k) X[k] = r[k];
+}
+template <int D>
+__device__ __forceinline__ void rec_u(const double *R, uint32_t i, double *u) {
+  const double *r = R + (size_t)i * (2 * D) + D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) u[k] = r[k];
+}
+template <int D>
+__device__ __forceinline__ void rec_set_u(double *R, uint32_t i, const double *u) {
+  double *r = R + (size_t)i * (2 * D) + D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) r[k] = u[k];
+}
+
 // Current reference position (pos, including a reset_partzero_y shift).
 template <int D>
-__device__ __forceinline__ void cur_pos(const NodeRec &r, const GrainDev &gp, double *X) {
-#pragma unroll
-  for (int k = 0; k < D; ++k) X[k] = r.X[k];
+__device__ __forceinline__ void cur_pos(const StepArgs &A, uint32_t i, const GrainDev &gp,
+                                        double *X) {
+  rec_X<D>(A, i, X);
   if (gp.flags & GF_YSHIFT) X[1] = X[1] + gp.yshift;
+}
+template <int D>
+__device__ __forceinline__ void load_vec(const double *p, uint32_t ns, uint32_t i, double *v) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) v[k] = p[(size_t)k * ns + i];
+}
+template <int D>
+__device__ __forceinline__ void store_vec(double *p, uint32_t ns, uint32_t i, const double *v) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) p[(size_t)k * ns + i] = v[k];
 }
 
 // bbox key arrays are [grain][2*D]: mins then maxes.
@@ -226,18 +322,31 @@ template <int D> struct Nbr {
   double u[D];
 };
 
-// Load the partner record of a column word.  Broken and padding slots load
-// the node's own record instead (always a valid address), so the bond loop
-// has no branches around its loads.
-template <int D>
-__device__ __forceinline__ void load_nbr(const NodeRec *R, uint32_t cw, uint32_t self,
-                                         Nbr<D> &nb) {
-  const NodeRec &r = R[(cw & BCOL_BROKEN) ? self : cw];
-#pragma unroll
-  for (int k = 0; k < D; ++k) nb.X[k] = r.X[k];
-  nb.V = r.X[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) nb.u[k] = r.u[k];
+// Load the partner data of a column word with 16-byte loads (3 in 3D, 2 in
+// 2D).  Broken and padding slots load the node's own record instead (always
+// a valid address), so the bond loop has no branches around its loads.
+// UVOL: the grain's nodes share one volume, passed in gvol.
+template <int D, bool UVOL>
+__device__ __forceinline__ void load_nbr(const double *R, const double *Vn, double gvol,
+                                         uint32_t cw, uint32_t self, Nbr<D> &nb) {
+  const uint32_t j = (cw & BCOL_BROKEN) ? self : cw;
+  const double2 *p = reinterpret_cast<const double2 *>(R + (size_t)j * (2 * D));
+  if constexpr (D == 3) {
+    const double2 a = p[0], b = p[1], c = p[2];
+    nb.X[0] = a.x;
+    nb.X[1] = a.y;
+    nb.X[2] = b.x;
+    nb.u[0] = b.y;
+    nb.u[1] = c.x;
+    nb.u[2] = c.y;
+  } else {
+    const double2 a = p[0], b = p[1];
+    nb.X[0] = a.x;
+    nb.X[1] = a.y;
+    nb.u[0] = b.x;
+    nb.u[1] = b.y;
+  }
+  nb.V = UVOL ? gvol : Vn[j];
 }
 
 // One bond of get_peridynamic_force (particle2.h:125-149) and its break test
@@ -250,18 +359,42 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
                                           const double *ui, double cnot, double snot, bool brk,
                                           uint32_t *slot, double *F) {
   const bool live = !(cw & BCOL_BROKEN);
-  double xi[D], eta[D];
+  double xi[D], eta[D], e[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) xi[q] = nb.X[q] - Xo[q];
-  const double xin = sqrt(red<D>(xi, xi));
+  const double xin2 = red<D>(xi, xi);
 #pragma unroll
   for (int q = 0; q < D; ++q) eta[q] = (nb.u[q] - ui[q]) + xi[q];
-  const double n = sqrt(red<D>(eta, eta));
+  const double n2 = red<D>(eta, eta);
+#if PG_FAST_DIVSQRT
+  // |xi| = sqrt(xin2), |eta| = sqrt(n2), s = (n - |xi|)/|xi|, e = eta/n,
+  // through the shortened sequences (one reciprocal for the three e_q).
+  double xin = sqrt_nr(xin2), n = sqrt_nr(n2);
+  double s = div_r(n - xin, xin, recip_nr(xin));
+  const double rn = recip_nr(n);
+#pragma unroll
+  for (int q = 0; q < D; ++q) e[q] = div_r(eta[q], n, rn);
+  bool ok = fast_sqrt_ok(xin2) && fast_sqrt_ok(n2) && fast_num_ok(n - xin);
+#pragma unroll
+  for (int q = 0; q < D; ++q) ok = ok && fast_num_ok(eta[q]);
+  if (live && !ok) { // outside the fast range: IEEE operators
+    xin = sqrt(xin2);
+    n = sqrt(n2);
+    s = (n - xin) / xin;
+#pragma unroll
+    for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
+  }
+#else
+  const double xin = sqrt(xin2);
+  const double n = sqrt(n2);
   const double s = (n - xin) / xin;
+#pragma unroll
+  for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
+#endif
   const double w = (cnot * s) * nb.V;
 #pragma unroll
   for (int q = 0; q < D; ++q) {
-    const double term = w * (eta[q] / n);
+    const double term = w * e[q];
     F[q] = F[q] + (live ? term : 0.0);
   }
   if (live && brk && s > snot) *slot = cw | BCOL_BROKEN;
@@ -276,20 +409,19 @@ __device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, u
                                                uint32_t g, const GrainDev &gp,
                                                const InstDev &ip, const double *xc,
                                                const double *Xc, bool selfc, bool stop,
-                                               const double *boxg, const NodeRec *Rr,
-                                               uint32_t &j, double *d, double &r,
-                                               double &R, double &K) {
+                                               const double *boxg, const double *U,
+                                               uint32_t &j, double *d, double &r, double &R,
+                                               double &K) {
   j = A.cl_j[c];
   const uint32_t gj = A.node_grain[j];
   R = ip.Rc;
   K = ip.Kn;
-  const NodeRec rj = Rr[j];
   double Xj[D];
   if (gj == g) {
     if (!selfc) return false;
     const int32_t bs = A.cl_slot[c];
     if (bs >= 0 && !(A.bcol[bs] & BCOL_BROKEN)) return false; // intact bond
-    cur_pos<D>(rj, gp, Xj);
+    cur_pos<D>(A, j, gp, Xj);
     double dX[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) dX[k] = Xc[k] - Xj[k];
@@ -307,10 +439,12 @@ __device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, u
     load_box<D>(A.bbox[t % 3], gj, boxj);
     if (!boxes_meet<D>(boxg, boxj, ip.Rc)) return false;
     const GrainDev gpj = A.grains[gj];
-    cur_pos<D>(rj, gpj, Xj);
+    cur_pos<D>(A, j, gpj, Xj);
   }
+  double uj[D];
+  rec_u<D>(U, j, uj);
 #pragma unroll
-  for (int k = 0; k < D; ++k) d[k] = (Xj[k] + rj.u[k]) - xc[k];
+  for (int k = 0; k < D; ++k) d[k] = (Xj[k] + uj[k]) - xc[k];
   r = sqrt(red<D>(d, d));
   return r <= R;
 }
@@ -366,9 +500,9 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 //   wall motion                           timeloop.h:866-878
 // followed by the position update of step t+1 (timeloop.h:661-669), the
 // grain bounding boxes of step t+1 (get_minmax, contact.h:1023-1062) and
-// the Verlet-skin check.  Reads u^t from rec[t&1] and v^{t-1} from
-// vel[(t+1)&1]; writes u^{t+1} to rec[(t+1)&1] and v^t to vel[t&1].
-template <int D, bool RECORD>
+// the Verlet-skin check.  Reads u^t from u[t&1] and v^{t-1} from
+// vel[(t+1)&1]; writes u^{t+1} to u[(t+1)&1] and v^t to vel[t&1].
+template <int D, bool RECORD, bool UVOL>
 __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
                                                                    const uint32_t t) {
   if (*A.violation <= t) return; // neighbour list stale since an earlier step
@@ -376,6 +510,7 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
   const uint32_t lb = xcd_block(blockIdx.x, nb);
   const uint32_t i = lb * blockDim.x + threadIdx.x;
   const uint32_t nthreads = nb * blockDim.x;
+  const uint32_t ns = A.ns;
 
   { // housekeeping: clear the box buffer of step t+2, move the walls
     unsigned long long *bz = A.bbox[(t + 2) % 3];
@@ -396,14 +531,14 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
 
   if (valid) {
     g = A.node_grain[i];
-    const NodeRec *Rr = A.rec[t & 1];
+    const double *U = A.rec[t & 1];
     double F[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) F[k] = 0.0;
 
     // ---- (a) PMB bond force with bond breaking.  Only what the bond loop
     // needs is live here; everything else is loaded after it.  The loop is
-    // software-pipelined: the partner records of the next two bonds and the
+    // software-pipelined: the partner data of the next two bonds and the
     // column words of the two after those are in flight while two bonds are
     // evaluated.
     {
@@ -414,56 +549,52 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
                          (iq.set_movable_grain == (int32_t)g && t >= iq.set_movable_step);
       if (mov_a) {
         double Xo[D], ui[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          Xo[k] = Rr[i].X[k];
-          ui[k] = Rr[i].u[k];
-        }
+        rec_X<D>(A, i, Xo);
+        rec_u<D>(U, i, ui);
         const double cnot = gq->cnot, snot = gq->snot;
+        const double gvol = UVOL ? gq->vol : 0.0;
         const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
         const uint32_t sl = i >> 6, lane = i & 63;
-      const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-      uint32_t *bc = A.bcol + base + lane;
-      // the column array has BCOL_TAIL spare slices after the last one, so
-      // reading up to 8 slots past a row's width stays in bounds
-      auto col = [&](uint32_t k) -> uint32_t {
-        const uint32_t w = bc[64u * k];
-        return k < width ? w : BCOL_PAD;
-      };
-      Nbr<D> ra0, ra1, rb0, rb1;
-      uint32_t ca0 = col(0), ca1 = col(1);
-      load_nbr<D>(Rr, ca0, i, ra0);
-      load_nbr<D>(Rr, ca1, i, ra1);
-      uint32_t cb0 = col(2), cb1 = col(3);
-      // Issue order matters: loads complete in order, so the column words
-      // (which have no dependences) go out before the record loads.
-      for (uint32_t k = 0; k < width; k += 4) {
-        const uint32_t na0 = col(k + 4), na1 = col(k + 5);
-        load_nbr<D>(Rr, cb0, i, rb0);
-        load_nbr<D>(Rr, cb1, i, rb1);
-        bond_term<D>(ca0, ra0, Xo, ui, cnot, snot, brk, bc + 64u * k, F);
-        bond_term<D>(ca1, ra1, Xo, ui, cnot, snot, brk, bc + 64u * (k + 1), F);
-        const uint32_t nb0 = col(k + 6), nb1 = col(k + 7);
-        load_nbr<D>(Rr, na0, i, ra0);
-        load_nbr<D>(Rr, na1, i, ra1);
-        bond_term<D>(cb0, rb0, Xo, ui, cnot, snot, brk, bc + 64u * (k + 2), F);
-        bond_term<D>(cb1, rb1, Xo, ui, cnot, snot, brk, bc + 64u * (k + 3), F);
-        ca0 = na0;
-        ca1 = na1;
-        cb0 = nb0;
-        cb1 = nb1;
-      }
+        const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+        // column words of this lane: group q (slots 4q..4q+3) at bc + 256*q
+        uint32_t *bc = A.bcol + base + BGROUP * lane;
+        // the column array has BCOL_TAIL spare slots after the last slice, so
+        // reading two groups past a row's end stays in bounds
+        auto colg = [&](uint32_t q) -> uint4 {
+          const uint4 w = *reinterpret_cast<const uint4 *>(bc + (SLICE * BGROUP) * q);
+          return (BGROUP * q < width) ? w : make_uint4(BCOL_PAD, BCOL_PAD, BCOL_PAD, BCOL_PAD);
+        };
+        Nbr<D> ra0, ra1, rb0, rb1;
+        uint4 cg = colg(0);
+        uint4 cn = colg(1);
+        load_nbr<D, UVOL>(U, A.V, gvol, cg.x, i, ra0);
+        load_nbr<D, UVOL>(U, A.V, gvol, cg.y, i, ra1);
+        // Issue order matters: loads complete in order, so the next column
+        // group (no dependences) goes out before the record loads that need
+        // the group loaded one iteration earlier.
+        for (uint32_t q = 0; BGROUP * q < width; ++q) {
+          uint32_t *sq = bc + (SLICE * BGROUP) * q;
+          load_nbr<D, UVOL>(U, A.V, gvol, cg.z, i, rb0);
+          load_nbr<D, UVOL>(U, A.V, gvol, cg.w, i, rb1);
+          bond_term<D>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F);
+          bond_term<D>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F);
+          const uint4 cnn = colg(q + 2);
+          load_nbr<D, UVOL>(U, A.V, gvol, cn.x, i, ra0);
+          load_nbr<D, UVOL>(U, A.V, gvol, cn.y, i, ra1);
+          bond_term<D>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F);
+          bond_term<D>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F);
+          cg = cn;
+          cn = cnn;
+        }
       }
     }
 
     // ---- everything after the bond sum
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
-    const NodeRec ri = Rr[i];
     double Xc[D], ui[D], xc[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) ui[k] = ri.u[k];
-    cur_pos<D>(ri, gp, Xc);
+    rec_u<D>(U, i, ui);
+    cur_pos<D>(A, i, gp, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
     const uint8_t info = A.node_info[i];
@@ -471,11 +602,8 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
     const double *vprev = A.vel[(t + 1) & 1];
     double *vcur = A.vel[t & 1];
     double vi[D], ai[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      vi[k] = vprev[(size_t)i * D + k];
-      ai[k] = A.acc[(size_t)i * D + k];
-    }
+    load_vec<D>(vprev, ns, i, vi);
+    load_vec<D>(A.acc, ns, i, ai);
     const uint32_t key = A.ckey[i];
     bool wall_hit = false;
     double wt[6][4];
@@ -497,13 +625,13 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
         for (uint32_t c = c0; c < c1; ++c) {
           uint32_t j;
           double d[D], r, R, K;
-          if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, boxg, Rr, j, d,
-                                  r, R, K))
+          if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, boxg, U, j, d, r, R,
+                                  K))
             continue;
           double w[D];
 #pragma unroll
-          for (int k = 0; k < D; ++k) w[k] = vi[k] - vprev[(size_t)j * D + k];
-          pair_force<D>(d, r, R, K, Rr[j].X[D], w, gp.rho, ip, F);
+          for (int k = 0; k < D; ++k) w[k] = vi[k] - vprev[(size_t)k * ns + j];
+          pair_force<D>(d, r, R, K, A.V[j], w, gp.rho, ip, F);
         }
       }
 
@@ -511,7 +639,7 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
       if (stop && (ip.flags & IF_WALL) && (info & NI_CONTACT)) {
         const double *wp = A.wallpos[t & 1] + (size_t)gp.inst * 6;
         if (!within_interior<D>(boxg, wp, ip.Rc)) {
-          const double vol = ri.X[D];
+          const double vol = A.V[i];
           if (D == 2) {
             wall_hit |= wall_side<D>(xc[1] - wp[3], 1, ip, gp.rho, vol, ip.speed[3], vi, F, wt[3]);
             wall_hit |= wall_side<D>(xc[0] - wp[0], 0, ip, gp.rho, vol, ip.speed[0], vi, F, wt[0]);
@@ -529,14 +657,16 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
       }
 
       // ---- external force (timeloop.h:776-786) and torque (:790-824)
+      double ex[D];
+      load_vec<D>(A.ext, ns, i, ex);
       if (ip.flags & IF_GRAD_EXT) {
         double sc = (double)t / (double)ip.extforce_maxstep;
         if (sc > 1) sc = 1;
 #pragma unroll
-        for (int k = 0; k < D; ++k) F[k] += sc * A.ext[(size_t)i * D + k];
+        for (int k = 0; k < D; ++k) F[k] += sc * ex[k];
       } else {
 #pragma unroll
-        for (int k = 0; k < D; ++k) F[k] += A.ext[(size_t)i * D + k];
+        for (int k = 0; k < D; ++k) F[k] += ex[k];
       }
       if (ip.flags & IF_TORQUE) {
         const double *cm = A.gmean + (size_t)g * 3;
@@ -577,11 +707,9 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
           ai[k] = 0.0;
         }
       }
-#pragma unroll
-      for (int k = 0; k < D; ++k) A.acc[(size_t)i * D + k] = ai[k];
+      store_vec<D>(A.acc, ns, i, ai);
     }
-#pragma unroll
-    for (int k = 0; k < D; ++k) vcur[(size_t)i * D + k] = vi[k];
+    store_vec<D>(vcur, ns, i, vi);
 
     // ---- position update of step t+1 (timeloop.h:661-669)
     double un[D];
@@ -592,26 +720,22 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
 #pragma unroll
       for (int k = 0; k < D; ++k) un[k] = ui[k];
     }
-    NodeRec *Rn = A.rec[(t + 1) & 1];
+    rec_set_u<D>(A.rec[(t + 1) & 1], i, un);
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      Rn[i].u[k] = un[k];
-      xn[k] = Xc[k] + un[k];
-    }
+    for (int k = 0; k < D; ++k) xn[k] = Xc[k] + un[k];
     if (info & NI_CONTACT) {
       double dd[D];
 #pragma unroll
-      for (int k = 0; k < D; ++k) dd[k] = xn[k] - A.xb[(size_t)i * D + k];
+      for (int k = 0; k < D; ++k) dd[k] = xn[k] - A.xb[(size_t)k * ns + i];
       viol = red<D>(dd, dd) > ip.skin_half2;
     }
     if (RECORD) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) A.force_out[(size_t)i * D + k] = F[k];
+      store_vec<D>(A.force_out, ns, i, F);
       if (key != 0xFFFFFFFFu) {
         A.wflag[key] = wall_hit ? 1 : 0;
         if (wall_hit) {
-          const int ns = (D == 2) ? 4 : 6;
-          for (int s = 0; s < ns; ++s)
+          const int nsd = (D == 2) ? 4 : 6;
+          for (int s = 0; s < nsd; ++s)
             for (int q = 0; q < 4; ++q) A.wterm[((size_t)key * 6 + s) * 4 + q] = wt[s][q];
         }
       }
@@ -628,6 +752,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < A.n_nodes;
+  const uint32_t ns = A.ns;
   uint32_t g = 0xFFFFFFFFu;
   double xn[D];
 #pragma unroll
@@ -636,38 +761,40 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     g = A.node_grain[i];
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
-    const NodeRec ri = A.rec[0][i];
-    double Xc[D], un[D];
-    cur_pos<D>(ri, gp, Xc);
+    double Xc[D], un[D], u0[D];
+    cur_pos<D>(A, i, gp, Xc);
+    rec_u<D>(A.rec[0], i, u0);
     const bool mov = grain_movable(gp, ip, g, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const double v = A.vel[0][(size_t)i * D + k];
-      const double a = A.acc[(size_t)i * D + k];
-      un[k] = mov ? ri.u[k] + ((ip.dt * v) + (ip.h * a)) : ri.u[k];
-      A.rec[1][i].u[k] = un[k];
+      const double v = A.vel[0][(size_t)k * ns + i];
+      const double a = A.acc[(size_t)k * ns + i];
+      un[k] = mov ? u0[k] + ((ip.dt * v) + (ip.h * a)) : u0[k];
       xn[k] = Xc[k] + un[k];
     }
+    rec_set_u<D>(A.rec[1], i, un);
   }
   wave_box_atomic<D>(A.bbox[1], g, valid, xn);
 }
 
-// mean_CurrPos of every grain (particle2.h:89-98), sequential per grain.
+// mean_CurrPos of every grain (particle2.h:89-98), summed sequentially in the
+// reference node order.
 template <int D>
 __global__ void k_grain_mean(const StepArgs A, const uint32_t t, double *gmean) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n_grains) return;
   const GrainDev gp = A.grains[g];
-  const NodeRec *Rr = A.rec[t & 1];
+  const double *U = A.rec[t & 1];
   double m[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) m[k] = 0.0;
-  for (uint32_t e = gp.node_begin; e < gp.node_end; ++e) { // reference node order
+  for (uint32_t e = gp.node_begin; e < gp.node_end; ++e) {
     const uint32_t n = A.ext2int[e];
-    double Xc[D];
-    cur_pos<D>(Rr[n], gp, Xc);
+    double Xc[D], un[D];
+    cur_pos<D>(A, n, gp, Xc);
+    rec_u<D>(U, n, un);
 #pragma unroll
-    for (int k = 0; k < D; ++k) m[k] += (Xc[k] + Rr[n].u[k]);
+    for (int k = 0; k < D; ++k) m[k] += (Xc[k] + un[k]);
   }
   const uint32_t nn = gp.node_end - gp.node_begin;
 #pragma unroll
@@ -738,29 +865,32 @@ __global__ __launch_bounds__(256) void k_reaction(const InstDev *insts, const do
 
 // ------------------------------------------------------- neighbour list
 
+template <int D>
+__device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_t i,
+                                          double *x, uint32_t &inst) {
+  if (!(A.node_info[i] & NI_CONTACT)) return false;
+  const GrainDev gp = A.grains[A.node_grain[i]];
+  double Xc[D], un[D];
+  cur_pos<D>(A, i, gp, Xc);
+  rec_u<D>(A.rec[t & 1], i, un);
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = Xc[k] + un[k];
+  inst = gp.inst;
+  return true;
+}
+
 // Per-instance box of contact nodes at step t (list build).
 template <int D>
 __global__ __launch_bounds__(256) void k_inst_box(const StepArgs A, const uint32_t t,
                                                   unsigned long long *ibox) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < A.n_nodes;
+  bool valid = false;
   uint32_t inst = 0xFFFFFFFFu;
   double x[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = 0.0;
-  if (valid) {
-    if (A.node_info[i] & NI_CONTACT) {
-      const GrainDev gp = A.grains[A.node_grain[i]];
-      const NodeRec r = A.rec[t & 1][i];
-      double Xc[D];
-      cur_pos<D>(r, gp, Xc);
-#pragma unroll
-      for (int k = 0; k < D; ++k) x[k] = Xc[k] + r.u[k];
-      inst = gp.inst;
-    } else {
-      valid = false;
-    }
-  }
+  if (i < A.n_nodes) valid = contact_x<D>(A, t, i, x, inst);
+  if (!valid) inst = 0xFFFFFFFFu;
   wave_box_atomic<D>(ibox, inst, valid, x);
 }
 
@@ -776,20 +906,6 @@ __device__ __forceinline__ void cell_coords(const GridDev &gd, const double *x, 
 template <int D> __device__ __forceinline__ uint32_t cell_lin(const GridDev &gd, const int *c) {
   if (D == 2) return gd.cell_begin + (uint32_t)c[1] * gd.dims[0] + c[0];
   return gd.cell_begin + ((uint32_t)c[2] * gd.dims[1] + c[1]) * gd.dims[0] + c[0];
-}
-
-template <int D>
-__device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_t i,
-                                          double *x, uint32_t &inst) {
-  if (!(A.node_info[i] & NI_CONTACT)) return false;
-  const GrainDev gp = A.grains[A.node_grain[i]];
-  const NodeRec r = A.rec[t & 1][i];
-  double Xc[D];
-  cur_pos<D>(r, gp, Xc);
-#pragma unroll
-  for (int k = 0; k < D; ++k) x[k] = Xc[k] + r.u[k];
-  inst = gp.inst;
-  return true;
 }
 
 template <int D>
@@ -822,7 +938,8 @@ __global__ void k_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t 
 }
 
 // Count (FILL=false) or write (FILL=true) the Verlet-list entries of node i:
-// every other contact node of the same instance within r2 at build time.
+// every other contact node of the same instance within the list radius at
+// build time.
 template <int D, bool FILL>
 __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
                        const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
@@ -871,8 +988,7 @@ __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
     cnt[i] = n;
     return;
   }
-#pragma unroll
-  for (int k = 0; k < D; ++k) xb[(size_t)i * D + k] = x[k];
+  store_vec<D>(xb, A.ns, i, x);
   // sort by the reference's accumulation order: own grain first, then by
   // contact key (grain, then node or boundary-list position)
   const uint32_t g = A.node_grain[i];
@@ -899,7 +1015,7 @@ __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
     int32_t slot = -1;
     if (A.node_grain[j] == g) {
       for (uint32_t k = 0; k < width; ++k) {
-        const uint32_t e = base + 64u * k + lane;
+        const uint32_t e = (uint32_t)bslot(base, lane, k);
         const uint32_t w = A.bcol[e];
         if (w == BCOL_PAD) break;
         if ((w & BCOL_INDEX) == j) {
@@ -928,18 +1044,18 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
   const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
   const bool stop = grain_stoppable(gp, ip, g, t);
   if (mov && (info & NI_CONTACT) && (selfc || stop)) {
-    const NodeRec *Rr = A.rec[t & 1];
-    const NodeRec ri = Rr[i];
-    double Xc[D], xc[D], box[2 * D];
-    cur_pos<D>(ri, gp, Xc);
+    const double *U = A.rec[t & 1];
+    double Xc[D], xc[D], box[2 * D], ui[D];
+    cur_pos<D>(A, i, gp, Xc);
+    rec_u<D>(U, i, ui);
 #pragma unroll
-    for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ri.u[k];
+    for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k];
     load_box<D>(A.bbox[t % 3], g, box);
     const uint32_t o = FILL ? ptr[i] : 0;
     for (uint32_t c = A.cl_ptr[i]; c < A.cl_ptr[i + 1]; ++c) {
       uint32_t j;
       double d[D], r, R, K;
-      if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, box, Rr, j, d, r, R, K))
+      if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, box, U, j, d, r, R, K))
         continue;
       if (FILL) {
         out[2 * (o + n)] = i;
@@ -957,19 +1073,42 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long a
   if (i < n) p[i] = ((i % period) < split) ? a : b;
 }
 
+// Diagnostics: the division and square root the step kernels use (the
+// shortened sequences inside their range, the IEEE operators outside),
+// applied elementwise, so tests can compare their rounding with the host's.
+__global__ void k_arith(uint64_t n, const double *a, const double *b, double *q, double *r) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#if PG_FAST_DIVSQRT
+  q[i] = div_rn(a[i], b[i]);
+  r[i] = sqrt_rn(a[i]);
+#else
+  q[i] = a[i] / b[i];
+  r[i] = sqrt(a[i]);
+#endif
+}
+
 // ------------------------------------------------------------- launchers
 
 static inline uint32_t nblk(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
 
+template <int D, bool RECORD>
+static void launch_step_t(bool uvol, const StepArgs &A, uint32_t t, uint32_t nb, int block,
+                          hipStream_t s) {
+  if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true>), dim3(nb), dim3(block), 0, s, A, t);
+  else hipLaunchKernelGGL((k_step<D, RECORD, false>), dim3(nb), dim3(block), 0, s, A, t);
+}
+
 hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int block,
                        hipStream_t s) {
   const uint32_t nb = nblk(A.n_nodes, block) > 0 ? nblk(A.n_nodes, block) : 1;
+  const bool uvol = A.uniform_vol != 0;
   if (dim == 2) {
-    if (record) hipLaunchKernelGGL((k_step<2, true>), dim3(nb), dim3(block), 0, s, A, t);
-    else hipLaunchKernelGGL((k_step<2, false>), dim3(nb), dim3(block), 0, s, A, t);
+    if (record) launch_step_t<2, true>(uvol, A, t, nb, block, s);
+    else launch_step_t<2, false>(uvol, A, t, nb, block, s);
   } else {
-    if (record) hipLaunchKernelGGL((k_step<3, true>), dim3(nb), dim3(block), 0, s, A, t);
-    else hipLaunchKernelGGL((k_step<3, false>), dim3(nb), dim3(block), 0, s, A, t);
+    if (record) launch_step_t<3, true>(uvol, A, t, nb, block, s);
+    else launch_step_t<3, false>(uvol, A, t, nb, block, s);
   }
   return hipGetLastError();
 }
@@ -1074,29 +1213,16 @@ hipError_t launch_pairs(int dim, bool fill, const StepArgs &A, uint32_t t, uint3
   return hipGetLastError();
 }
 
-hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp,
-                              size_t *tmp_bytes, hipStream_t s) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, s);
-}
-
-} // namespace pg
-
-namespace pg {
-
-// Diagnostics: the division and square root the step kernels use, applied
-// elementwise, so tests can compare their rounding with the host's.
-__global__ void k_arith(uint64_t n, const double *a, const double *b, double *q, double *r) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  q[i] = a[i] / b[i];
-  r[i] = sqrt(a[i]);
-}
-
 hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
                         hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_arith, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, a, b, q, r);
   return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp,
+                              size_t *tmp_bytes, hipStream_t s) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, s);
 }
 
 } // namespace pg
