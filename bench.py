@@ -3,12 +3,18 @@ workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
 moving top wall, gravity), one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --copy-calibration MIB   # plain device copy, for PMC scaling
 
 Prints one JSON line.  `value` is node-steps/s with the state resident in
 HBM; the timed region is K steps enqueued through pg_step, bracketed by HIP
 events on the stepper's own stream.  `roofline` prices the fused step
-kernel against HBM bandwidth with the algorithmic bytes of one step,
-N*(8*(8d+1)+16) + 5*M (N nodes, M directed bond slots, d=3).
+kernel against HBM bandwidth: the algorithmic bytes of one step,
+N*(8*(8d+1)+16) + 5*M (N nodes, M directed bond slots, d=3), over the
+step kernel's average duration, which libpg measures with HIP events
+around its runs of step launches (pg_kernel_time).  `traffic` is the HBM
+bytes per step-kernel launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes,
+scaled by the same counters on a plain copy of known size; it is read from
+profiles/step_kernel_pmc.json when that file was measured on this workload.
 `cpu_baseline` times the FP64 CPU restatement of the reference algorithm
 (the parity oracle) on a few steps of the same workload.
 """
@@ -22,6 +28,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "node-steps/s at 256k-node 3D; batched/plain speedup; adjoint\u00f7forward time"
+PMC_FILE = os.path.join(ROOT, "profiles", "step_kernel_pmc.json")
 
 
 def algorithmic_bytes(n_nodes, n_slots, dim):
@@ -46,6 +54,34 @@ def cpu_baseline(setup, params, threads, timed_steps=2):
     }
 
 
+def copy_calibration(mib):
+    """Copy a buffer of `mib` MiB device-to-device a few times (torch's copy
+    kernel), so rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch can be
+    compared with the known mib*1024 KiB read and written."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_kib": mib * 1024, "dispatches": 5}), flush=True)
+
+
+def measured_traffic(workload, n_nodes, n_slots):
+    """HBM bytes per step-kernel launch from the committed PMC summary, if
+    it was measured on this exact workload; else None."""
+    try:
+        with open(PMC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (rec.get("workload"), rec.get("nodes"), rec.get("directed_bonds")) != (
+            workload, n_nodes, n_slots):
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,7 +90,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -76,21 +116,27 @@ def main():
         g.step(args.warmup)
         g.sync()
         st0 = g.stats()
+        k0 = g.kernel_time()
         g.event(0)
         g.step(args.steps)
         g.event(1)
         ms = g.elapsed_ms(0, 1)
         g.sync()
         st1 = g.stats()
+        k1 = g.kernel_time()
     n_nodes, n_slots = setup.n_nodes, setup.n_slots()
     ms_per_step = ms / args.steps
     value = n_nodes * args.steps / (ms / 1e3)
     bytes_step = algorithmic_bytes(n_nodes, n_slots, dim)
-    achieved = bytes_step / (ms_per_step / 1e3) / 1e9
+    k_launches = k1[1] - k0[1]
+    kernel_us = (k1[0] - k0[0]) * 1e3 / k_launches if k_launches else ms_per_step * 1e3
+    achieved = bytes_step / (kernel_us / 1e6) / 1e9
     print(f"bench: {args.steps} steps in {ms:.2f} ms ({ms_per_step * 1e3:.2f} us/step), "
+          f"step kernel {kernel_us:.2f} us over {k_launches} launches, "
           f"list rebuilds {st1['rebuilds'] - st0['rebuilds']}, "
           f"skipped steps {st1['skipped_steps'] - st0['skipped_steps']}",
           file=sys.stderr, flush=True)
+    workload = "3d_compression_64_spheres"
 
     cpu = None
     if not args.no_cpu:
@@ -98,7 +144,7 @@ def main():
         cpu = cpu_baseline(setup, params, threads)
 
     line = {
-        "metric": "node-steps/s at 256k-node 3D",
+        "metric": METRIC,
         "value": value,
         "unit": "node-steps/s",
         "n_gpus": 1,
@@ -111,7 +157,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded jittered-lattice spheres)",
         "config": {
-            "workload": "3d_compression_64_spheres",
+            "workload": workload,
             "nodes": n_nodes,
             "directed_bonds": n_slots,
             "grains": setup.n_grains,
@@ -125,8 +171,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_step": bytes_step,
+            "traffic": measured_traffic(workload, n_nodes, n_slots),
+            "algorithmic_bytes_per_launch": bytes_step,
+            "kernel_us": kernel_us,
+            "kernel_launches": k_launches,
         },
         "cpu_baseline": cpu,
     }

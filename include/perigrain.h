@@ -194,7 +194,7 @@ int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries,
 
 /* Tuning knobs (used by the autotuner).  skin_factor: Verlet skin as a
  * multiple of the contact radius; block: threads per block of the step
- * kernel (64..1024, multiple of 64).  Takes effect at the next rebuild. */
+ * kernel (64, 128 or 256).  The contact list is rebuilt at once. */
 int pg_set_tuning(pg_handle *h, double skin_factor, int block);
 
 /* Capture chunks of steps into hipGraphs (1) or launch each step (0). */
@@ -205,6 +205,14 @@ int pg_set_graph(pg_handle *h, int use_graph);
  * (waits for the later one). */
 int pg_event_record(pg_handle *h, int slot);
 int pg_event_elapsed(pg_handle *h, int a, int b, float *ms);
+
+/* Device time spent in step-kernel launches since the handle was created:
+ * pg_step brackets each run of back-to-back step launches with HIP events on
+ * the handle's stream and adds up the elapsed times of runs that no
+ * neighbour-list rebuild cut short.  *launches counts the step launches of
+ * those runs, so *ms / *launches is the average step-kernel duration
+ * (including any per-step grain-mean launches when torque is on). */
+int pg_kernel_time(pg_handle *h, double *ms, uint64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,9 @@ struct pg_handle {
   size_t scan_tmp_bytes = 0;
   uint32_t *h_pin = nullptr;
   hipEvent_t ev[8] = {};
+  hipEvent_t kev[2] = {}; // step-kernel timing (pg_kernel_time)
+  double kernel_ms = 0.0;
+  uint64_t kernel_launches = 0;
   uint64_t rebuilds = 0, skipped = 0, list_entries = 0;
   StepArgs A{};
 };
@@ -614,6 +617,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
+  for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&h->kev[k]));
   h->ns = (uint32_t)(((N + SLICE - 1) / SLICE) * SLICE);
   if (h->ns == 0) h->ns = SLICE;
   const size_t ND = (size_t)h->ns * D; // component-major [D][ns]
@@ -723,6 +727,8 @@ void destroy_impl(pg_handle *h) {
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   for (int k = 0; k < 8; ++k)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+  for (int k = 0; k < 2; ++k)
+    if (h->kev[k]) (void)hipEventDestroy(h->kev[k]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -733,14 +739,22 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
   if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
   while (h->t < t_end) {
     const uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
+    HIPCHK(hipEventRecord(h->kev[0], h->stream));
     for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
       if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
       HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
     }
+    HIPCHK(hipEventRecord(h->kev[1], h->stream));
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     const uint32_t v = h->h_pin[0];
+    if (v > h->t + chunk) { // no launch of this run returned early
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+      h->kernel_ms += ms;
+      h->kernel_launches += chunk;
+    }
     if (v <= h->t + chunk) {
       if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid before the batch started");
       h->skipped += (uint64_t)(h->t + chunk) - v + 1;
@@ -1028,6 +1042,13 @@ int pg_event_elapsed(pg_handle *h, int a, int b, float *ms) {
   if (!h || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(PG_ERR_INVALID, "bad event slot");
   HIPCHK(hipEventSynchronize(h->ev[b]));
   HIPCHK(hipEventElapsedTime(ms, h->ev[a], h->ev[b]));
+  return PG_OK;
+}
+
+int pg_kernel_time(pg_handle *h, double *ms, uint64_t *launches) {
+  if (!h) return fail(PG_ERR_INVALID, "null handle");
+  if (ms) *ms = h->kernel_ms;
+  if (launches) *launches = h->kernel_launches;
   return PG_OK;
 }
 

@@ -135,3 +135,10 @@ class Stepper:
         ms = ctypes.c_float()
         check(lib().pg_event_elapsed(self._h, a, b, ctypes.byref(ms)))
         return float(ms.value)
+
+    def kernel_time(self):
+        """(milliseconds, launches) of step-kernel runs so far (pg_kernel_time)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        check(lib().pg_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return float(ms.value), int(n.value)
