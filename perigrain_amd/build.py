@@ -1,11 +1,19 @@
-"""Build step of the package: compiles libpg.so for gfx950 in-tree.
+"""Build step of the package, in-tree:
+
+  perigrain_amd/libpg.so          the HIP stepper (gfx950)
+  perigrain_amd/libpgio.so        config / HDF5 file I/O of the drop-in
+  perigrain_amd/bin/simulate2d    drop-ins for the reference's executables
+  perigrain_amd/bin/simulate3d
 
     python -m perigrain_amd.build
 
 Every HIP source is compiled with -ffp-contract=off (each +,-,*,/ a single
 IEEE binary64 rounding, as in the reference's FMA-free build) and with packed
-FP32 instructions disabled.
+FP32 instructions disabled.  The HDF5 C library comes from the image
+(/opt/conda); bin/hdf5/ holds a link to it so that the run-time search path
+of the built files names that one library and nothing else of /opt/conda.
 """
+import glob
 import os
 import subprocess
 import sys
@@ -23,6 +31,10 @@ HIP_FLAGS = [
 ]
 
 LIBPG_SOURCES = ["pg_kernels.hip", "pg_api.hip"]
+HOST = os.path.join(HERE, "host")
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+HDF5_PREFIXES = [os.environ.get("PG_HDF5_PREFIX", ""), "/opt/conda", "/usr"]
 
 
 def _run(cmd, cwd=None):
@@ -45,8 +57,46 @@ def build_libpg():
     return out
 
 
+def find_hdf5():
+    """(include dir, path of the versioned libhdf5 shared object)."""
+    for pre in HDF5_PREFIXES:
+        if not pre:
+            continue
+        inc = os.path.join(pre, "include")
+        libs = sorted(glob.glob(os.path.join(pre, "lib", "libhdf5.so.[0-9]*")),
+                      key=len)
+        if os.path.exists(os.path.join(inc, "hdf5.h")) and libs:
+            return inc, libs[0]
+    raise RuntimeError("HDF5 C library (hdf5.h, libhdf5.so.N) not found; set PG_HDF5_PREFIX")
+
+
+def build_io():
+    inc, lib = find_hdf5()
+    hdir = os.path.join(BIN, "hdf5")
+    os.makedirs(hdir, exist_ok=True)
+    link = os.path.join(hdir, os.path.basename(lib))
+    if os.path.islink(link) or os.path.exists(link):
+        os.remove(link)
+    os.symlink(lib, link)
+    soname = os.path.basename(lib)
+    cxx = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE, "-I", inc]
+    out_io = os.path.join(HERE, "libpgio.so")
+    _run(cxx + ["-shared", os.path.join(HOST, "pgio.cpp"), "-o", out_io + ".tmp",
+                "-L", hdir, "-l:" + soname, "-Wl,-rpath,$ORIGIN/bin/hdf5"])
+    os.replace(out_io + ".tmp", out_io)
+    outs = [out_io]
+    for dim in (2, 3):
+        exe = os.path.join(BIN, f"simulate{dim}d")
+        _run(cxx + [f"-DPG_DIM={dim}", os.path.join(HOST, "simulate.cpp"), "-o", exe + ".tmp",
+                    "-L", HERE, "-lpg", "-lpgio", "-L", hdir, "-l:" + soname,
+                    "-Wl,-rpath,$ORIGIN/..:$ORIGIN/hdf5"])
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
+
+
 def build_all():
-    return [build_libpg()]
+    return [build_libpg()] + build_io()
 
 
 if __name__ == "__main__":

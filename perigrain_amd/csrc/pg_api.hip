@@ -109,6 +109,9 @@ struct pg_handle {
   double kernel_ms = 0.0;
   uint64_t kernel_launches = 0;
   uint64_t rebuilds = 0, skipped = 0, list_entries = 0;
+  // host copy of the bond table, valid for step bcol_host_t
+  std::vector<uint32_t> bcol_host;
+  uint32_t bcol_host_t = 0xFFFFFFFFu;
   StepArgs A{};
 };
 
@@ -302,6 +305,19 @@ int build_list(pg_handle *h, uint32_t ts) {
   HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
   h->list_entries = total;
   h->rebuilds += 1;
+  return PG_OK;
+}
+
+// Bond table of the current step on the host (one copy per step, shared by
+// the per-grain readers).
+int bond_status(pg_handle *h, const std::vector<uint32_t> *&st) {
+  if (h->bcol_host_t != h->t || h->bcol_host.size() != h->E) {
+    h->bcol_host.resize(h->E);
+    HIPCHK(hipSetDevice(h->device));
+    RC(d2h(h, h->bcol_host.data(), h->d_bcol, h->E));
+    h->bcol_host_t = h->t;
+  }
+  st = &h->bcol_host;
   return PG_OK;
 }
 
@@ -898,8 +914,9 @@ int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
 
 int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst) {
   if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  std::vector<uint32_t> st(h->E);
-  RC(d2h(h, st.data(), h->d_bcol, h->E));
+  const std::vector<uint32_t> *stp = nullptr;
+  RC(bond_status(h, stp));
+  const std::vector<uint32_t> &st = *stp;
   const InstHost &ih = h->inst[inst];
   for (uint32_t e = ih.node_begin; e < ih.node_end; ++e) { // reference order
     const uint32_t i = h->inv[e];
@@ -919,9 +936,9 @@ int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_
     return fail(PG_ERR_INVALID, "bad grain");
   const uint32_t gg = ih.grain_begin + grain;
   const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
-  // status of this grain's rows only
-  std::vector<uint32_t> st(h->E);
-  RC(d2h(h, st.data(), h->d_bcol, h->E));
+  const std::vector<uint32_t> *stp = nullptr;
+  RC(bond_status(h, stp));
+  const std::vector<uint32_t> &st = *stp;
   uint64_t r = 0;
   for (uint32_t ge = a; ge < b; ++ge) { // reference node order
     const uint32_t i = h->inv[ge];
