@@ -2,7 +2,7 @@
 workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
 moving top wall, gravity), one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
     python bench.py --copy-calibration MIB   # plain device copy, for PMC scaling
 
 Prints one JSON line.  `value` is node-steps/s with the state resident in
@@ -17,6 +17,13 @@ scaled by the same counters on a plain copy of known size; it is read from
 profiles/step_kernel_pmc.json when that file was measured on this workload.
 `cpu_baseline` times the FP64 CPU restatement of the reference algorithm
 (the parity oracle) on a few steps of the same workload.
+
+`batched` measures the batched form on 256 independent 2D gravity-settle
+instances (~2.3M nodes, uneven sizes): one handle holding all of them
+(graph-replayed step chunks, autotuned skin and block) against the plain
+per-instance loop on the same GPU (one handle per instance, same settings,
+each advanced the same number of steps in turn).  Both are timed on the host
+clock over whole pg_step calls.
 """
 import argparse
 import json
@@ -68,6 +75,54 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_kib": mib * 1024, "dispatches": 5}), flush=True)
 
 
+def batched_bench(steps, seed, n_instances=256):
+    from perigrain_amd import Stepper, setups
+    t0 = time.perf_counter()
+    S, P = setups.batched_settle(n_instances, seed)
+    n_nodes = sum(s.n_nodes for s in S)
+    print(f"bench: batched workload built in {time.perf_counter() - t0:.1f} s: "
+          f"{n_instances} instances, {n_nodes} nodes", file=sys.stderr, flush=True)
+    with Stepper(S, P, device=0) as b:
+        b.set_graph(True)
+        skin, block = b.autotune(trial_steps=64)
+        b.step(64)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t_batched = time.perf_counter() - t0
+    print(f"bench: batched {steps} steps in {t_batched * 1e3:.1f} ms (skin {skin}, block {block})",
+          file=sys.stderr, flush=True)
+    handles = [Stepper(s, p, device=0) for s, p in zip(S, P)]
+    try:
+        for h in handles:
+            h.set_graph(True)
+            h.set_tuning(skin, block)
+            h.step(64)
+            h.sync()
+        t0 = time.perf_counter()
+        for h in handles:
+            h.step(steps)
+        for h in handles:
+            h.sync()
+        t_plain = time.perf_counter() - t0
+    finally:
+        for h in handles:
+            h.close()
+    print(f"bench: plain per-instance loop {steps} steps in {t_plain * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {
+        "workload": f"{n_instances} x 2D gravity settle (uneven instances)",
+        "nodes": n_nodes,
+        "instances": n_instances,
+        "steps": steps,
+        "batched_node_steps_per_s": n_nodes * steps / t_batched,
+        "plain_loop_node_steps_per_s": n_nodes * steps / t_plain,
+        "speedup": t_plain / t_batched,
+        "tuning": {"skin_factor": skin, "block": block, "graph": True},
+    }
+
+
 def measured_traffic(workload, n_nodes, n_slots):
     """HBM bytes per step-kernel launch from the committed PMC summary, if
     it was measured on this exact workload; else None."""
@@ -88,6 +143,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--batched-steps", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -138,6 +195,10 @@ def main():
           file=sys.stderr, flush=True)
     workload = "3d_compression_64_spheres"
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(args.batched_steps, args.seed)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
@@ -177,6 +238,7 @@ def main():
             "kernel_launches": k_launches,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -197,7 +197,19 @@ int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries,
  * kernel (64, 128 or 256).  The contact list is rebuilt at once. */
 int pg_set_tuning(pg_handle *h, double skin_factor, int block);
 
-/* Capture chunks of steps into hipGraphs (1) or launch each step (0). */
+/* Autotuner (no reference counterpart): runs the next trial_steps steps
+ * once per candidate configuration -- Verlet skin 0.25, 0.5 or 1.0 times
+ * the contact radius, step-kernel blocks of 64, 128 or 256 threads -- timing
+ * each on the host clock including neighbour-list rebuilds, and keeps the
+ * fastest.  The trials are ordinary steps of the simulation (9*trial_steps
+ * in total): skin and block size change which candidate pairs are listed and
+ * how threads are grouped, never a result. */
+int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *block);
+
+/* Replay whole chunks of step launches (64 steps that do not include the
+ * recording step) from a captured hipGraph (1), or launch each step (0).
+ * The step index of a replayed launch comes from a device-side base that
+ * the host sets before each replay.  Results are the same either way. */
 int pg_set_graph(pg_handle *h, int use_graph);
 
 /* HIP events on the handle's stream (slots 0..7), for timing by callers:

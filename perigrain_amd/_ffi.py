@@ -190,6 +190,7 @@ def lib():
         "pg_stats": [H, _u64p, _u64p, _u64p],
         "pg_set_tuning": [H, ctypes.c_double, ctypes.c_int],
         "pg_set_graph": [H, ctypes.c_int],
+        "pg_autotune": [H, ctypes.c_uint32, _dp, ctypes.POINTER(ctypes.c_int)],
         "pg_event_record": [H, ctypes.c_int],
         "pg_event_elapsed": [H, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
         "pg_kernel_time": [H, _dp, _u64p],

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -104,6 +105,12 @@ struct pg_handle {
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   uint32_t *h_pin = nullptr;
+  // graph replay of whole chunks of step launches (pg_set_graph)
+  bool use_graph = false;
+  uint32_t *tbase = nullptr; // device-side base step of a replayed chunk
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t args_gen = 0, graph_gen = 0; // graph is valid while graph_gen == args_gen
   hipEvent_t ev[8] = {};
   hipEvent_t kev[2] = {}; // step-kernel timing (pg_kernel_time)
   double kernel_ms = 0.0;
@@ -159,6 +166,8 @@ template <class T> int d2h(pg_handle *h, T *dst, const T *src, size_t n) {
 
 void fill_args(pg_handle *h) {
   StepArgs &A = h->A;
+  h->args_gen++;
+  A.tbase = nullptr;
   A.rec[0] = h->rec[0];
   A.rec[1] = h->rec[1];
   A.V = h->V;
@@ -665,6 +674,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->wallpos[1], (size_t)n * 6));
   RC(dalloc(h, h->gmean, h->any_torque ? (size_t)G * 3 : 1));
   RC(dalloc(h, h->violation, 1));
+  RC(dalloc(h, h->tbase, 1));
   RC(dalloc(h, h->wterm, (size_t)(CK ? CK : 1) * 24));
   RC(dalloc(h, h->wflag, CK ? CK : 1));
   RC(dalloc(h, h->reaction, (size_t)n * 18));
@@ -740,6 +750,8 @@ void destroy_impl(pg_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void *p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   for (int k = 0; k < 8; ++k)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
@@ -749,16 +761,54 @@ void destroy_impl(pg_handle *h) {
   delete h;
 }
 
+// Capture one chunk of check_every step launches (no recording step) whose
+// step index is the device-side base plus 1..check_every.
+int ensure_graph(pg_handle *h) {
+  if (h->gexec && h->graph_gen == h->args_gen) return PG_OK;
+  if (h->gexec) HIPCHK(hipGraphExecDestroy(h->gexec));
+  if (h->graph) HIPCHK(hipGraphDestroy(h->graph));
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  StepArgs G = h->A;
+  G.tbase = h->tbase;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (uint32_t k = 1; k <= h->check_every && e == hipSuccess; ++k) {
+    if (h->any_torque) e = launch_grain_mean(h->dim, G, k, h->gmean, h->stream);
+    if (e == hipSuccess) e = launch_step(h->dim, false, G, k, h->block, h->stream);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+  h->graph = g;
+  HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  h->graph_gen = h->args_gen;
+  return PG_OK;
+}
+
 int run_steps(pg_handle *h, uint32_t nsteps) {
   if (nsteps == 0) return PG_OK;
   const uint32_t t_end = h->t + nsteps;
   if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
   while (h->t < t_end) {
     const uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
+    // a whole chunk without the recording step can be replayed from a graph
+    const bool replay = h->use_graph && chunk == h->check_every && h->t + chunk < t_end;
+    if (replay) {
+      RC(ensure_graph(h));
+      h->h_pin[8] = h->t; // read by the copy below, before the chunk's final sync
+      HIPCHK(hipMemcpyAsync(h->tbase, &h->h_pin[8], sizeof(uint32_t), hipMemcpyHostToDevice,
+                            h->stream));
+    }
     HIPCHK(hipEventRecord(h->kev[0], h->stream));
-    for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
-      if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
-      HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+    if (replay) {
+      HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    } else {
+      for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
+        if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
+        HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+      }
     }
     HIPCHK(hipEventRecord(h->kev[1], h->stream));
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1031,6 +1081,7 @@ int pg_set_tuning(pg_handle *h, double skin_factor, int block) {
   HIPCHK(hipSetDevice(h->device));
   h->block = block;
   h->skin_factor = skin_factor;
+  h->args_gen++; // a captured chunk holds the old block size
   for (uint32_t s = 0; s < h->I; ++s) {
     InstHost &ih = h->inst[s];
     ih.skin = skin_factor * (ih.Rc > 0 ? ih.Rc : 0.0);
@@ -1042,9 +1093,35 @@ int pg_set_tuning(pg_handle *h, double skin_factor, int block) {
   return build_list(h, h->t + 1);
 }
 
+int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *block) {
+  if (!h || trial_steps == 0) return fail(PG_ERR_INVALID, "null handle or zero trial steps");
+  static const double skins[] = {0.25, 0.5, 1.0};
+  static const int blocks[] = {64, 128, 256};
+  double best = 1e300, best_skin = h->skin_factor;
+  int best_block = h->block;
+  for (double sk : skins)
+    for (int bl : blocks) {
+      RC(pg_set_tuning(h, sk, bl));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      RC(run_steps(h, trial_steps));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (dt < best) {
+        best = dt;
+        best_skin = sk;
+        best_block = bl;
+      }
+    }
+  RC(pg_set_tuning(h, best_skin, best_block));
+  if (skin_factor) *skin_factor = best_skin;
+  if (block) *block = best_block;
+  return PG_OK;
+}
+
 int pg_set_graph(pg_handle *h, int use_graph) {
   if (!h) return fail(PG_ERR_INVALID, "null handle");
-  (void)use_graph;
+  h->use_graph = use_graph != 0;
   return PG_OK;
 }
 

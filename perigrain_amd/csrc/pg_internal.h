@@ -129,6 +129,7 @@ struct StepArgs {
   double *wallpos[2];          // [insts][6]
   const double *gmean;         // [grains][3] mean CurrPos (torque)
   uint32_t *violation;         // first step whose positions broke the skin
+  const uint32_t *tbase;       // graph replays: base step added to the launch's step argument
   double *force_out;           // [dim][ns], record steps
   double *wterm;               // record steps: [ckeys][6][4]
   uint8_t *wflag;              // record steps: [ckeys]

@@ -119,6 +119,12 @@ __device__ __forceinline__ bool grain_stoppable(const GrainDev &gp, const InstDe
          (ip.set_stoppable_grain == (int32_t)g && t >= ip.set_stoppable_step);
 }
 
+// Step index of a launch: the argument itself, or, for launches replayed
+// from a captured graph, the argument added to the device-side base step.
+__device__ __forceinline__ uint32_t step_index(const StepArgs &A, uint32_t t_arg) {
+  return A.tbase ? *A.tbase + t_arg : t_arg;
+}
+
 // Record accessors: records are [node][2*D], X then u.
 template <int D>
 __device__ __forceinline__ void rec_X(const StepArgs &A, uint32_t i, double *X) {
@@ -504,7 +510,8 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 // vel[(t+1)&1]; writes u^{t+1} to u[(t+1)&1] and v^t to vel[t&1].
 template <int D, bool RECORD, bool UVOL>
 __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
-                                                                   const uint32_t t) {
+                                                                   const uint32_t t_arg) {
+  const uint32_t t = step_index(A, t_arg);
   if (*A.violation <= t) return; // neighbour list stale since an earlier step
   const uint32_t nb = gridDim.x;
   const uint32_t lb = xcd_block(blockIdx.x, nb);
@@ -780,7 +787,8 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
 // mean_CurrPos of every grain (particle2.h:89-98), summed sequentially in the
 // reference node order.
 template <int D>
-__global__ void k_grain_mean(const StepArgs A, const uint32_t t, double *gmean) {
+__global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gmean) {
+  const uint32_t t = step_index(A, t_arg);
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n_grains) return;
   const GrainDev gp = A.grains[g];

@@ -258,16 +258,18 @@ def make_grain_3d(R, h, rng, contact_radius):
 def random_placement(radii, lo, hi, rng, gap, max_tries=20000):
     """Random non-overlapping circles/spheres (circumradius + gap) in [lo, hi]."""
     dim = len(lo)
-    centers = []
-    for R in radii:
+    C = np.zeros((len(radii), dim))
+    Rs = np.zeros(len(radii))
+    for k, R in enumerate(radii):
         for _ in range(max_tries):
             c = rng.uniform(np.asarray(lo) + R + gap, np.asarray(hi) - R - gap)
-            if all(np.linalg.norm(c - c2) >= R + R2 + gap for c2, R2 in centers):
-                centers.append((c, R))
+            if k == 0 or np.all(np.linalg.norm(C[:k] - c, axis=1) >= Rs[:k] + R + gap):
+                C[k] = c
+                Rs[k] = R
                 break
         else:
             raise RuntimeError("random placement failed; box too small")
-    return [c for c, _ in centers]
+    return [C[k].copy() for k in range(len(radii))]
 
 
 # --------------------------------------------------------------- workloads
@@ -366,7 +368,8 @@ def settle_instance_2d(seed):
     """One instance of the batched workload: grains U[20,150], shapes from
     {disk, square, plus, annulus}, radius U[0.5,1.2] mm, 50-400 nodes per
     grain (drawn as 50*8**(u**3)), gravity settle in a box of width
-    U[20,60] mm and height twice that.  dt=2e-8, 5000 steps."""
+    U[20,60] mm and height twice that (1.5x taller again whenever the random
+    placement jams).  dt=2e-8, 5000 steps."""
     rng = np.random.default_rng(seed)
     G = int(rng.integers(20, 151))
     width = float(rng.uniform(20e-3, 60e-3))
@@ -379,7 +382,13 @@ def settle_instance_2d(seed):
     Rc = dmed / 5
     grains = [make_grain_2d(sh, R, h, rng, Rc) for sh, R, h in zip(shapes, radii, hs)]
     _, cmed, _ = material_2d(dmed)
-    centers = random_placement(radii, [0.0, 0.0], [width, height], rng, Rc)
+    while True:  # a taller box when the random placement jams
+        try:
+            centers = random_placement(radii, [0.0, 0.0], [width, height], rng, Rc,
+                                       max_tries=2000)
+            break
+        except RuntimeError:
+            height *= 1.5
     s = assemble(2, grains, centers, [np.zeros(2)] * G, Rc, cmed, 0.8, 0.3, 9.81, 1,
                  [0.0, width, height, 0.0])
     p = Params(dt=2e-8, timesteps=5000, modulo=1000, enable_fracture=1)

@@ -136,6 +136,14 @@ class Stepper:
         check(lib().pg_event_elapsed(self._h, a, b, ctypes.byref(ms)))
         return float(ms.value)
 
+    def autotune(self, trial_steps=64):
+        """Pick the fastest (skin factor, block) by running 9*trial_steps
+        real steps (pg_autotune); returns the choice."""
+        sk = ctypes.c_double()
+        bl = ctypes.c_int()
+        check(lib().pg_autotune(self._h, int(trial_steps), ctypes.byref(sk), ctypes.byref(bl)))
+        return float(sk.value), int(bl.value)
+
     def kernel_time(self):
         """(milliseconds, launches) of step-kernel runs so far (pg_kernel_time)."""
         ms = ctypes.c_double()
