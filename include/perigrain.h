@@ -212,6 +212,25 @@ int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *bl
  * the host sets before each replay.  Results are the same either way. */
 int pg_set_graph(pg_handle *h, int use_graph);
 
+/* Differentiable form (no reference counterpart).  pg_adjoint_forward runs
+ * nsteps steps of a one-instance handle, recording the wall reaction of
+ * every step into reaction_history ([nsteps][PG_MAX_SIDES][dim], the
+ * get_reaction() layout of each step) and checkpointing the state every
+ * checkpoint_every steps.  pg_adjoint_backward takes the cotangent of that
+ * history (dL/dR, same layout) and returns in grad[3] the derivatives of L
+ * with respect to
+ *   grad[0]: a common scale of every grain's micromodulus cnot (at 1), which
+ *            also scales the self-contact stiffness cnot/r;
+ *   grad[1]: the normal contact stiffness (node-node and wall contact);
+ *   grad[2]: the friction coefficient.
+ * Bond status, contact pairs and every branch are frozen to the forward
+ * pass; segments between checkpoints are recomputed from the checkpoints.
+ * Requires no torque, no timed toggles and no clamped nodes.  After the
+ * backward pass the handle is back in the state after the recorded steps. */
+int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
+                       double *reaction_history);
+int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad);
+
 /* HIP events on the handle's stream (slots 0..7), for timing by callers:
  * record an event, then read the elapsed milliseconds between two slots
  * (waits for the later one). */

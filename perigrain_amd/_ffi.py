@@ -194,6 +194,8 @@ def lib():
         "pg_event_record": [H, ctypes.c_int],
         "pg_event_elapsed": [H, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
         "pg_kernel_time": [H, _dp, _u64p],
+        "pg_adjoint_forward": [H, ctypes.c_uint32, ctypes.c_uint32, _dp],
+        "pg_adjoint_backward": [H, _dp, _dp],
         "pg_device_count": [ctypes.POINTER(ctypes.c_int)],
         "pg_check_arith": [ctypes.c_int, ctypes.c_uint64, _dp, _dp, _dp, _dp],
     }

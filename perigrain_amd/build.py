@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-fPIC", "-Wall", "-Wno-unused-result",
 ]
 
-LIBPG_SOURCES = ["pg_kernels.hip", "pg_api.hip"]
+LIBPG_SOURCES = ["pg_kernels.hip", "pg_adjoint.hip", "pg_api.hip"]
 HOST = os.path.join(HERE, "host")
 BIN = os.path.join(HERE, "bin")
 CXX = os.environ.get("CXX", "g++")

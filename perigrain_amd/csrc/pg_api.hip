@@ -116,6 +116,29 @@ struct pg_handle {
   double kernel_ms = 0.0;
   uint64_t kernel_launches = 0;
   uint64_t rebuilds = 0, skipped = 0, list_entries = 0;
+  // differentiable runs (pg_adjoint_*)
+  bool any_clamped = false;
+  double *hist = nullptr; // while recording: [steps][insts][18] wall reaction of every step
+  uint32_t hist_t0 = 0;
+  struct Adj {
+    bool recorded = false;
+    uint32_t t0 = 0, T = 0, K = 0, nck = 0;
+    uint32_t *break_step = nullptr; // [E + tail] step at which each slot's bond broke
+    double *hist = nullptr;         // [T][18]
+    // checkpoint k: state after step t0 + k*K (the last one: after t0 + T)
+    double *ck_rec = nullptr, *ck_vel = nullptr, *ck_acc = nullptr, *ck_wall = nullptr;
+    unsigned long long *ck_box = nullptr;
+    // snapshots of one segment, one per step
+    double *su = nullptr, *sv = nullptr, *swall = nullptr;
+    unsigned long long *sbox = nullptr;
+    double *Pu = nullptr, *Pv = nullptr, *Pa = nullptr, *lf[2] = {nullptr, nullptr};
+    double *gpart = nullptr, *lR = nullptr;
+    unsigned long long *dmax = nullptr;
+    // what the readers see after the recorded pass: u^T, force, reaction
+    double *fin_rec = nullptr, *fin_force = nullptr, *fin_reaction = nullptr;
+    unsigned long long *fin_box = nullptr;
+    size_t cap_ck = 0, cap_seg = 0, cap_T = 0;
+  } adj;
   // host copy of the bond table, valid for step bcol_host_t
   std::vector<uint32_t> bcol_host;
   uint32_t bcol_host_t = 0xFFFFFFFFu;
@@ -202,6 +225,7 @@ void fill_args(pg_handle *h) {
   A.n_insts = h->I;
   A.any_torque = h->any_torque ? 1 : 0;
   A.uniform_vol = h->uniform_vol ? 1 : 0;
+  A.break_step = h->adj.break_step;
 }
 
 int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
@@ -224,10 +248,15 @@ int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
   return PG_OK;
 }
 
-// Build the contact Verlet list from the positions of step ts
-// (u^ts in u[ts & 1]).
-int build_list(pg_handle *h, uint32_t ts) {
+// Build the contact Verlet list from the positions of step ts (u^ts in
+// rec[ts & 1], or in rec_override when given).
+int build_list(pg_handle *h, uint32_t ts, const double *rec_override = nullptr) {
   const int D = h->dim;
+  auto args = [&]() {
+    StepArgs A2 = h->A;
+    if (rec_override) A2.rec[ts & 1] = const_cast<double *>(rec_override);
+    return A2;
+  };
   if (!h->any_contact) {
     HIPCHK(hipMemsetAsync(h->cl_ptr, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
     HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
@@ -235,7 +264,7 @@ int build_list(pg_handle *h, uint32_t ts) {
   }
   // 1. per-instance box of contact nodes
   HIPCHK(launch_fill_box(h->ibox, (size_t)h->I * 2 * D, D, h->stream));
-  HIPCHK(launch_inst_box(D, h->A, ts, h->ibox, h->stream));
+  HIPCHK(launch_inst_box(D, args(), ts, h->ibox, h->stream));
   std::vector<unsigned long long> kb((size_t)h->I * 2 * D);
   RC(d2h(h, kb.data(), h->ibox, kb.size()));
   // 2. per-instance grids
@@ -290,13 +319,13 @@ int build_list(pg_handle *h, uint32_t ts) {
   }
   // 3. counting sort of contact nodes into cells
   HIPCHK(hipMemsetAsync(h->cell_cnt, 0, (cells + 1) * sizeof(uint32_t), h->stream));
-  HIPCHK(launch_cell_count(D, h->A, ts, h->grids, h->cell_of, h->cell_cnt, h->stream));
+  HIPCHK(launch_cell_count(D, args(), ts, h->grids, h->cell_of, h->cell_cnt, h->stream));
   RC(scan(h, h->cell_cnt, h->cell_start, cells + 1));
   HIPCHK(hipMemsetAsync(h->cell_cur, 0, (cells + 1) * sizeof(uint32_t), h->stream));
   HIPCHK(launch_cell_fill(h->N, h->cell_of, h->cell_start, h->cell_cur, h->cell_nodes, h->stream));
   // 4. count, scan, fill the per-node lists
   HIPCHK(hipMemsetAsync(h->cl_cnt, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
-  HIPCHK(launch_list(D, false, h->A, ts, h->grids, h->cell_start, h->cell_nodes, h->cl_cnt,
+  HIPCHK(launch_list(D, false, args(), ts, h->grids, h->cell_start, h->cell_nodes, h->cl_cnt,
                      nullptr, nullptr, nullptr, nullptr, h->stream));
   RC(scan(h, h->cl_cnt, h->cl_ptr, (size_t)h->N + 1));
   uint32_t total = 0;
@@ -309,7 +338,7 @@ int build_list(pg_handle *h, uint32_t ts) {
     RC(dalloc(h, h->cl_slot, h->cl_cap));
     fill_args(h);
   }
-  HIPCHK(launch_list(D, true, h->A, ts, h->grids, h->cell_start, h->cell_nodes, nullptr,
+  HIPCHK(launch_list(D, true, args(), ts, h->grids, h->cell_start, h->cell_nodes, nullptr,
                      h->cl_ptr, h->cl_j, h->cl_slot, h->xb, h->stream));
   HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
   h->list_entries = total;
@@ -446,6 +475,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         const uint32_t cn = st.clamped_nodes[c];
         if (cn >= (uint64_t)(b - a)) return fail(PG_ERR_INVALID, "clamped node out of range");
         node_info[ih.node_begin + a + cn] |= NI_CLAMPED;
+        h->any_clamped = true;
       }
       GrainDev &gd = grains[gg];
       std::memset(&gd, 0, sizeof(gd));
@@ -794,7 +824,8 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
   while (h->t < t_end) {
     const uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
     // a whole chunk without the recording step can be replayed from a graph
-    const bool replay = h->use_graph && chunk == h->check_every && h->t + chunk < t_end;
+    const bool replay =
+        h->use_graph && !h->hist && chunk == h->check_every && h->t + chunk < t_end;
     if (replay) {
       RC(ensure_graph(h));
       h->h_pin[8] = h->t; // read by the copy below, before the chunk's final sync
@@ -807,7 +838,10 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     } else {
       for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
         if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
-        HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+        HIPCHK(launch_step(h->dim, s == t_end || h->hist, h->A, s, h->block, h->stream));
+        if (h->hist) // wall reaction of every step (differentiable runs)
+          HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag,
+                                 h->hist + (size_t)(s - h->hist_t0 - 1) * h->I * 18, h->stream));
       }
     }
     HIPCHK(hipEventRecord(h->kev[1], h->stream));
@@ -831,6 +865,131 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     h->t += chunk;
   }
   HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag, h->reaction, h->stream));
+  return PG_OK;
+}
+
+// ------------------------------------------------------------ adjoint
+
+size_t rec_len(const pg_handle *h) { return (size_t)h->ns * 2 * h->dim; }
+size_t vec_len(const pg_handle *h) { return (size_t)h->ns * h->dim; }
+size_t box_len(const pg_handle *h) { return (size_t)h->G * 2 * h->dim; }
+
+template <class T> int d2d(pg_handle *h, T *dst, const T *src, size_t n) {
+  if (n == 0) return PG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+  return PG_OK;
+}
+
+// Save the state after step t (everything step t+1 reads) as checkpoint k.
+int ck_save(pg_handle *h, uint32_t k, uint32_t t) {
+  auto &a = h->adj;
+  RC(d2d(h, a.ck_rec + k * rec_len(h), h->rec[(t + 1) & 1], rec_len(h)));
+  RC(d2d(h, a.ck_vel + k * vec_len(h), h->vel[t & 1], vec_len(h)));
+  RC(d2d(h, a.ck_acc + k * vec_len(h), h->acc, vec_len(h)));
+  RC(d2d(h, a.ck_wall + (size_t)k * h->I * 6, h->wallpos[(t + 1) & 1], (size_t)h->I * 6));
+  RC(d2d(h, a.ck_box + k * box_len(h), h->bbox[(t + 1) % 3], box_len(h)));
+  return PG_OK;
+}
+
+// Restore checkpoint k (the state after step t) into the live buffers: bond
+// status from the recorded break steps, a fresh box buffer for step t+2 and
+// a new neighbour list.
+int ck_restore(pg_handle *h, uint32_t k, uint32_t t) {
+  auto &a = h->adj;
+  RC(d2d(h, h->rec[(t + 1) & 1], a.ck_rec + k * rec_len(h), rec_len(h)));
+  RC(d2d(h, h->vel[t & 1], a.ck_vel + k * vec_len(h), vec_len(h)));
+  RC(d2d(h, h->acc, a.ck_acc + k * vec_len(h), vec_len(h)));
+  RC(d2d(h, h->wallpos[(t + 1) & 1], a.ck_wall + (size_t)k * h->I * 6, (size_t)h->I * 6));
+  RC(d2d(h, h->bbox[(t + 1) % 3], a.ck_box + k * box_len(h), box_len(h)));
+  HIPCHK(launch_fill_box(h->bbox[(t + 2) % 3], box_len(h), h->dim, h->stream));
+  HIPCHK(launch_adj_restore_bcol(h->d_bcol, a.break_step, h->E, t, h->stream));
+  h->t = t;
+  h->bcol_host_t = 0xFFFFFFFFu;
+  return build_list(h, t + 1);
+}
+
+int adj_alloc(pg_handle *h, uint32_t T, uint32_t K) {
+  auto &a = h->adj;
+  const uint32_t nck = (T + K - 1) / K;
+  if (!a.break_step) {
+    RC(dalloc(h, a.break_step, h->E + BCOL_TAIL));
+    fill_args(h);
+  }
+  if ((size_t)nck + 1 > a.cap_ck) {
+    RC(dfree(h, a.ck_rec));
+    RC(dfree(h, a.ck_vel));
+    RC(dfree(h, a.ck_acc));
+    RC(dfree(h, a.ck_wall));
+    RC(dfree(h, a.ck_box));
+    a.cap_ck = (size_t)nck + 1;
+    RC(dalloc(h, a.ck_rec, a.cap_ck * rec_len(h)));
+    RC(dalloc(h, a.ck_vel, a.cap_ck * vec_len(h)));
+    RC(dalloc(h, a.ck_acc, a.cap_ck * vec_len(h)));
+    RC(dalloc(h, a.ck_wall, a.cap_ck * h->I * 6));
+    RC(dalloc(h, a.ck_box, a.cap_ck * box_len(h)));
+  }
+  if (K > a.cap_seg) {
+    RC(dfree(h, a.su));
+    RC(dfree(h, a.sv));
+    RC(dfree(h, a.swall));
+    RC(dfree(h, a.sbox));
+    a.cap_seg = K;
+    RC(dalloc(h, a.su, a.cap_seg * rec_len(h)));
+    RC(dalloc(h, a.sv, a.cap_seg * vec_len(h)));
+    RC(dalloc(h, a.swall, a.cap_seg * h->I * 6));
+    RC(dalloc(h, a.sbox, a.cap_seg * box_len(h)));
+  }
+  if (T > a.cap_T) {
+    RC(dfree(h, a.hist));
+    RC(dfree(h, a.lR));
+    a.cap_T = T;
+    RC(dalloc(h, a.hist, a.cap_T * h->I * 18));
+    RC(dalloc(h, a.lR, a.cap_T * h->I * 6 * h->dim));
+  }
+  if (!a.Pu) {
+    RC(dalloc(h, a.Pu, vec_len(h)));
+    RC(dalloc(h, a.Pv, vec_len(h)));
+    RC(dalloc(h, a.Pa, vec_len(h)));
+    RC(dalloc(h, a.lf[0], vec_len(h)));
+    RC(dalloc(h, a.lf[1], vec_len(h)));
+    RC(dalloc(h, a.gpart, (size_t)((h->N + 255) / 256 + 1) * 3));
+    RC(dalloc(h, a.dmax, 1));
+    RC(dalloc(h, a.fin_rec, rec_len(h)));
+    RC(dalloc(h, a.fin_force, vec_len(h)));
+    RC(dalloc(h, a.fin_reaction, (size_t)h->I * 18));
+    RC(dalloc(h, a.fin_box, box_len(h)));
+  }
+  a.nck = nck;
+  return PG_OK;
+}
+
+// Forward recompute of steps c+1 .. c+n from checkpoint state, saving before
+// every step t what the adjoint of step t reads: u^t, v^{t-1}, the wall
+// positions and the grain boxes of step t.
+int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
+  auto &a = h->adj;
+  const uint32_t end = c + n;
+  while (h->t < end) {
+    for (uint32_t t = h->t + 1; t <= end; ++t) {
+      const size_t q = t - c - 1;
+      RC(d2d(h, a.su + q * rec_len(h), h->rec[t & 1], rec_len(h)));
+      RC(d2d(h, a.sv + q * vec_len(h), h->vel[(t + 1) & 1], vec_len(h)));
+      RC(d2d(h, a.swall + q * h->I * 6, h->wallpos[t & 1], (size_t)h->I * 6));
+      RC(d2d(h, a.sbox + q * box_len(h), h->bbox[t % 3], box_len(h)));
+      HIPCHK(launch_step(h->dim, false, h->A, t, h->block, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const uint32_t v = h->h_pin[0];
+    if (v <= end) {
+      if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid in the recompute");
+      h->t = v - 1;
+      RC(build_list(h, v));
+      continue;
+    }
+    h->t = end;
+  }
   return PG_OK;
 }
 
@@ -1116,6 +1275,124 @@ int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *bl
   RC(pg_set_tuning(h, best_skin, best_block));
   if (skin_factor) *skin_factor = best_skin;
   if (block) *block = best_block;
+  return PG_OK;
+}
+
+int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
+                       double *reaction_history) {
+  if (!h || !reaction_history) return fail(PG_ERR_INVALID, "null argument");
+  if (nsteps == 0 || checkpoint_every == 0) return fail(PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
+  if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
+  const InstDev &id = h->inst_dev[0];
+  if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped)
+    return fail(PG_ERR_INVALID,
+                "the adjoint supports no torque, no timed movable/stoppable toggles and no clamped nodes");
+  HIPCHK(hipSetDevice(h->device));
+  auto &a = h->adj;
+  RC(adj_alloc(h, nsteps, checkpoint_every));
+  HIPCHK(launch_adj_init_break(h->d_bcol, a.break_step, h->E, h->stream));
+  a.t0 = h->t;
+  a.T = nsteps;
+  a.K = checkpoint_every;
+  a.recorded = false;
+  RC(ck_save(h, 0, h->t));
+  h->hist = a.hist;
+  h->hist_t0 = a.t0;
+  int rc = PG_OK;
+  for (uint32_t k = 0; k < a.nck && rc == PG_OK; ++k) {
+    const uint32_t n = std::min(a.K, a.T - k * a.K);
+    rc = run_steps(h, n);
+    if (rc == PG_OK) rc = ck_save(h, k + 1, h->t);
+  }
+  h->hist = nullptr;
+  RC(rc);
+  RC(d2d(h, a.fin_rec, h->rec[h->t & 1], rec_len(h)));
+  RC(d2d(h, a.fin_force, h->force_out, vec_len(h)));
+  RC(d2d(h, a.fin_reaction, h->reaction, (size_t)h->I * 18));
+  RC(d2d(h, a.fin_box, h->bbox[h->t % 3], box_len(h)));
+  const int D = h->dim;
+  std::vector<double> hh((size_t)nsteps * 18);
+  RC(d2h(h, hh.data(), a.hist, hh.size()));
+  for (size_t t = 0; t < nsteps; ++t)
+    for (int sd = 0; sd < PG_MAX_SIDES; ++sd)
+      for (int k = 0; k < D; ++k)
+        reaction_history[(t * PG_MAX_SIDES + sd) * D + k] = hh[t * 18 + sd * 3 + k];
+  a.recorded = true;
+  return PG_OK;
+}
+
+int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad) {
+  if (!h || !reaction_bar || !grad) return fail(PG_ERR_INVALID, "null argument");
+  auto &a = h->adj;
+  if (!a.recorded) return fail(PG_ERR_STATE, "no recorded forward pass (pg_adjoint_forward)");
+  HIPCHK(hipSetDevice(h->device));
+  const int D = h->dim;
+  RC(h2d(h, a.lR, reaction_bar, (size_t)a.T * PG_MAX_SIDES * D));
+  HIPCHK(hipMemsetAsync(a.Pu, 0, vec_len(h) * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(a.Pv, 0, vec_len(h) * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(a.Pa, 0, vec_len(h) * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(a.lf[0], 0, vec_len(h) * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(a.lf[1], 0, vec_len(h) * sizeof(double), h->stream));
+  const uint32_t blocks = (h->N + 255) / 256;
+  HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
+  InstHost &ih = h->inst[0];
+  const double skin0 = ih.skin;
+  int rc = PG_OK;
+  for (int64_t k = (int64_t)a.nck - 1; k >= 0 && rc == PG_OK; --k) {
+    const uint32_t c = a.t0 + (uint32_t)k * a.K;
+    const uint32_t n = std::min(a.K, a.T - (uint32_t)k * a.K);
+    rc = ck_restore(h, (uint32_t)k, c);
+    if (rc == PG_OK) rc = adj_recompute(h, c, n);
+    if (rc != PG_OK) break;
+    // one candidate list for the whole segment: pairs within the contact
+    // radius at any of its steps are within radius + twice the largest
+    // displacement at its first step
+    const unsigned long long zero = 0;
+    HIPCHK(hipMemcpyAsync(a.dmax, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(launch_adj_maxdisp(D, h->A, a.su, n, a.dmax, h->stream));
+    unsigned long long dk = 0;
+    RC(d2h(h, &dk, a.dmax, 1));
+    const double dmax = dkey_inv(dk);
+    ih.skin = 2.0 * dmax * (1.0 + 1e-6) + 1e-12 * ih.Rc;
+    rc = build_list(h, c + 1, a.su);
+    ih.skin = skin0;
+    if (rc != PG_OK) break;
+    for (uint32_t t = c + n; t >= c + 1; --t) {
+      const size_t q = t - c - 1;
+      AdjArgs B;
+      B.A = h->A;
+      B.su = a.su + q * rec_len(h);
+      B.sv = a.sv + q * vec_len(h);
+      B.swall = a.swall + q * h->I * 6;
+      B.sbox = a.sbox + q * box_len(h);
+      B.lR = a.lR + (size_t)(t - a.t0 - 1) * PG_MAX_SIDES * D;
+      B.lf_in = a.lf[t & 1];
+      B.lf_out = a.lf[(t - 1) & 1];
+      B.Pu = a.Pu;
+      B.Pv = a.Pv;
+      B.Pa = a.Pa;
+      B.gpart = a.gpart;
+      hipError_t e = launch_adj_step(D, B, t, blocks, h->stream);
+      if (e != hipSuccess) {
+        rc = fail(PG_ERR_DEVICE, std::string("adjoint step: ") + hipGetErrorString(e));
+        break;
+      }
+    }
+  }
+  ih.skin = skin0;
+  RC(rc);
+  std::vector<double> gp((size_t)blocks * 3);
+  RC(d2h(h, gp.data(), a.gpart, gp.size()));
+  grad[0] = grad[1] = grad[2] = 0.0;
+  for (uint32_t b = 0; b < blocks; ++b)
+    for (int q = 0; q < 3; ++q) grad[q] += gp[(size_t)b * 3 + q];
+  // back to the state after the recorded pass
+  RC(ck_restore(h, a.nck, a.t0 + a.T));
+  RC(d2d(h, h->rec[h->t & 1], a.fin_rec, rec_len(h)));
+  RC(d2d(h, h->force_out, a.fin_force, vec_len(h)));
+  RC(d2d(h, h->reaction, a.fin_reaction, (size_t)h->I * 18));
+  RC(d2d(h, h->bbox[h->t % 3], a.fin_box, box_len(h)));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return PG_OK;
 }
 

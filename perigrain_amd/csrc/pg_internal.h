@@ -130,6 +130,7 @@ struct StepArgs {
   const double *gmean;         // [grains][3] mean CurrPos (torque)
   uint32_t *violation;         // first step whose positions broke the skin
   const uint32_t *tbase;       // graph replays: base step added to the launch's step argument
+  uint32_t *break_step;        // differentiable runs: step at which each slot's bond broke
   double *force_out;           // [dim][ns], record steps
   double *wterm;               // record steps: [ckeys][6][4]
   uint8_t *wflag;              // record steps: [ckeys]
@@ -139,6 +140,22 @@ struct StepArgs {
   uint32_t n_insts;
   uint32_t any_torque;
   uint32_t uniform_vol; // every grain has GF_UNIFORM_VOL
+};
+
+// Adjoint of one time step (differentiable runs).  State of step t as the
+// forward pass saw it, the cotangents carried backwards, and the parameter
+// partial sums.  Node vectors are [dim][ns] unless noted.
+struct AdjArgs {
+  StepArgs A;                  // static data; A.cl_* is the segment's candidate list
+  const double *su;            // [n][2*dim] records holding u^t
+  const double *sv;            // v^{t-1}
+  const double *swall;         // [insts][6] wall positions of step t
+  const unsigned long long *sbox; // [grains][2*dim] grain boxes of step t
+  const double *lR;            // [insts][6][dim] cotangent of the wall reaction of step t
+  const double *lf_in;         // cotangent of the force density of step t
+  double *lf_out;              // ... of step t-1 (written)
+  double *Pu, *Pv, *Pa;        // cotangents of u^{t+1}, v^t, a^t (partial), updated to step t-1
+  double *gpart;               // [blocks][3]: d/d(c scale), d/dKn, d/dmu partial sums
 };
 
 // Order-preserving map of an IEEE double onto uint64 (for atomic min/max).

@@ -31,6 +31,14 @@ hipError_t launch_list(int dim, bool fill, const StepArgs &A, uint32_t t, const 
                        hipStream_t s);
 hipError_t launch_pairs(int dim, bool fill, const StepArgs &A, uint32_t t, uint32_t *cnt,
                         const uint32_t *ptr, uint32_t *out, hipStream_t s);
+// adjoint (pg_adjoint.hip)
+hipError_t launch_adj_step(int dim, const AdjArgs &B, uint32_t t, uint32_t blocks, hipStream_t s);
+hipError_t launch_adj_maxdisp(int dim, const StepArgs &A, const double *su, uint32_t steps,
+                              unsigned long long *out, hipStream_t s);
+hipError_t launch_adj_restore_bcol(uint32_t *bcol, const uint32_t *break_step, uint64_t n,
+                                   uint32_t t, hipStream_t s);
+hipError_t launch_adj_init_break(const uint32_t *bcol, uint32_t *break_step, uint64_t n,
+                                 hipStream_t s);
 hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
                         hipStream_t s);
 // hipcub exclusive sum; call with tmp=nullptr to size *tmp_bytes.

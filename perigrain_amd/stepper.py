@@ -144,6 +144,27 @@ class Stepper:
         check(lib().pg_autotune(self._h, int(trial_steps), ctypes.byref(sk), ctypes.byref(bl)))
         return float(sk.value), int(bl.value)
 
+    def adjoint_forward(self, nsteps, checkpoint_every=100):
+        """Run nsteps recorded steps (pg_adjoint_forward); returns the wall
+        reaction of every step, shape (nsteps, sides, dim)."""
+        ns = 4 if self.dim == 2 else 6
+        out = np.zeros((int(nsteps), 6, self.dim))
+        check(lib().pg_adjoint_forward(self._h, int(nsteps), int(checkpoint_every),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out[:, :ns].copy()
+
+    def adjoint_backward(self, reaction_bar):
+        """Cotangent of the recorded reaction history -> derivatives with
+        respect to (micromodulus scale, normal stiffness, friction)."""
+        ns = 4 if self.dim == 2 else 6
+        rb = np.asarray(reaction_bar, dtype=np.float64)
+        full = np.zeros((rb.shape[0], 6, self.dim))
+        full[:, :ns] = rb
+        g = np.zeros(3)
+        check(lib().pg_adjoint_backward(self._h, full.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return g
+
     def kernel_time(self):
         """(milliseconds, launches) of step-kernel runs so far (pg_kernel_time)."""
         ms = ctypes.c_double()
