@@ -2,7 +2,7 @@
 workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
 moving top wall, gravity), one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-adjoint]
     python bench.py --copy-calibration MIB   # plain device copy, for PMC scaling
 
 Prints one JSON line.  `value` is node-steps/s with the state resident in
@@ -17,6 +17,12 @@ scaled by the same counters on a plain copy of known size; it is read from
 profiles/step_kernel_pmc.json when that file was measured on this workload.
 `cpu_baseline` times the FP64 CPU restatement of the reference algorithm
 (the parity oracle) on a few steps of the same workload.
+
+`adjoint` measures the differentiable form on the 2D compression of 64
+grains (~15k nodes, 2000 steps, checkpoints every 100): the plain forward
+run, the recorded forward pass and the adjoint (backward) pass, each timed on
+the host clock after a warm-up run, and the adjoint gradient of the top-wall
+reaction misfit against central finite differences (relative step 1e-6).
 
 `batched` measures the batched form on 256 independent 2D gravity-settle
 instances (~2.3M nodes, uneven sizes): one handle holding all of them
@@ -123,6 +129,83 @@ def batched_bench(steps, seed, n_instances=256):
     }
 
 
+def adjoint_bench(steps=2000, ck=100):
+    from perigrain_amd import Stepper, diffrun
+    run = diffrun.compression_2d_run(steps=steps, checkpoint_every=ck)
+    theta0, theta_t = (1.5e9, 1.0, 0.25), (1.0e9, 1.2, 0.4)
+    target, st = run.forward(*theta_t)
+    st[0].close()
+    s, p = run.make_setup(*theta0)
+    times = {}
+    for rep in range(2):  # the first repetition warms up
+        with Stepper(s, p, device=0) as g:
+            g.sync()
+            t0 = time.perf_counter()
+            g.step(steps)
+            g.sync()
+            t_fwd = time.perf_counter() - t0
+        with Stepper(s, p, device=0) as g:
+            g.sync()
+            t0 = time.perf_counter()
+            hist = g.adjoint_forward(steps, ck)
+            t_rec = time.perf_counter() - t0
+            loss, bar = diffrun.top_wall_misfit(hist, target)
+            t0 = time.perf_counter()
+            raw = g.adjoint_backward(bar)
+            t_bwd = time.perf_counter() - t0
+        times = {"forward_ms": t_fwd * 1e3, "recorded_forward_ms": t_rec * 1e3,
+                 "backward_ms": t_bwd * 1e3}
+    Kn = s.normal_stiffness
+    grad = [(raw[0] + Kn * raw[1]) / theta0[0], Kn * raw[1] / theta0[1], raw[2]]
+    fd = []
+    for k in range(3):
+        h = 1e-6 * theta0[k]
+        vals = []
+        for sgn in (1, -1):
+            th = list(theta0)
+            th[k] += sgn * h
+            hh, stt = run.forward(*th)
+            stt[0].close()
+            vals.append(diffrun.top_wall_misfit(hh, target)[0])
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    rel = [abs(a - b) / abs(b) for a, b in zip(grad, fd)]
+    # FP64 CPU reference gradient: the oracle on dual numbers (same frozen
+    # branches), contracted with its own misfit cotangent
+    import numpy as np
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    Rc, dRc = oracle.reaction_history_tangent(s, p, steps, threads=threads)
+    t_cpu = time.perf_counter() - t0
+    bar_c = np.zeros_like(Rc)
+    bar_c[:, 2] = 2.0 * (Rc[:, 2] - target[:, 2])
+    raw_c = np.einsum("tsk,tskq->q", bar_c, dRc)
+    grad_c = [(raw_c[0] + Kn * raw_c[1]) / theta0[0], Kn * raw_c[1] / theta0[1], raw_c[2]]
+    rel_c = [abs(a - b) / abs(b) for a, b in zip(grad, grad_c)]
+    print(f"bench: adjoint forward {times['forward_ms']:.1f} ms, recorded "
+          f"{times['recorded_forward_ms']:.1f} ms, backward {times['backward_ms']:.1f} ms, "
+          f"FD rel err {rel}, CPU dual-number rel err {rel_c} ({t_cpu:.1f} s)",
+          file=sys.stderr, flush=True)
+    return {
+        "workload": "2D compression, 64 grains, top wall -0.5 m/s",
+        "nodes": s.n_nodes,
+        "steps": steps,
+        "checkpoint_every": ck,
+        **times,
+        "adjoint_over_forward": times["backward_ms"] / times["forward_ms"],
+        "recorded_plus_adjoint_over_forward":
+            (times["recorded_forward_ms"] + times["backward_ms"]) / times["forward_ms"],
+        "theta": {"K": theta0[0], "contact_scale": theta0[1], "mu": theta0[2]},
+        "loss": loss,
+        "gradient": grad,
+        "central_difference": fd,
+        "fd_rel_err": rel,
+        "cpu_dual_gradient": grad_c,
+        "cpu_dual_rel_err": rel_c,
+        "cpu_dual_threads": threads,
+    }
+
+
 def measured_traffic(workload, n_nodes, n_slots):
     """HBM bytes per step-kernel launch from the committed PMC summary, if
     it was measured on this exact workload; else None."""
@@ -145,6 +228,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batched-steps", type=int, default=512)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -199,6 +283,10 @@ def main():
     if not args.no_batched:
         batched = batched_bench(args.batched_steps, args.seed)
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench()
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
@@ -239,6 +327,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

@@ -50,6 +50,9 @@ def lib():
         "oracle_read_conn": [H, ctypes.c_int, u32p, ctypes.c_uint64, u64p],
         "oracle_read_contacts": [H, u32p, ctypes.c_uint64, u64p],
         "oracle_max_threads": [],
+        "oracle_create_tangent": [ctypes.POINTER(_ffi.PgSetup), ctypes.POINTER(_ffi.PgParams),
+                                  ctypes.POINTER(H)],
+        "oracle_read_wall_tangent": [H, dp, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -69,13 +72,14 @@ def _check(rc):
 class Oracle:
     """The reference time loop on the CPU, same interface as Stepper."""
 
-    def __init__(self, setup, params, threads=1):
+    def __init__(self, setup, params, threads=1, tangent=False):
         from perigrain_amd import _ffi
         L = lib()
         st, self._keep = _ffi.setup_struct(setup)
         pp = _ffi.params_struct(params)
         h = ctypes.c_void_p()
-        _check(L.oracle_create(ctypes.byref(st), ctypes.byref(pp), ctypes.byref(h)))
+        create = L.oracle_create_tangent if tangent else L.oracle_create
+        _check(create(ctypes.byref(st), ctypes.byref(pp), ctypes.byref(h)))
         self._h = h
         self.dim = int(setup.dim)
         self.threads = int(threads)
@@ -117,6 +121,17 @@ class Oracle:
         ns = 4 if self.dim == 2 else 6
         return info[:ns].copy(), react.reshape(6, self.dim)[:ns].copy()
 
+    def wall_tangent(self):
+        """Reaction of the last step (sides, dim) and, for a tangent-mode
+        oracle, its derivatives (sides, dim, 3) along (cnot scale,
+        normal_stiffness, friction_coefficient)."""
+        ns = 4 if self.dim == 2 else 6
+        r = np.zeros(ns * self.dim)
+        dr = np.zeros(ns * self.dim * 3)
+        _check(lib().oracle_read_wall_tangent(self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                              dr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return r.reshape(ns, self.dim), dr.reshape(ns, self.dim, 3)
+
     def bond_intact(self):
         _, _, ns = self.sizes()
         out = np.empty(ns, np.uint8)
@@ -139,6 +154,23 @@ class Oracle:
             _check(lib().oracle_read_contacts(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                               n.value, ctypes.byref(n)))
         return out
+
+
+def reaction_history_tangent(setup, params, steps, threads=1):
+    """FP64 CPU reference (dual numbers): the wall reaction of every step,
+    (steps, sides, dim), and its derivatives along (cnot scale,
+    normal_stiffness, friction_coefficient), (steps, sides, dim, 3)."""
+    o = Oracle(setup, params, threads=threads, tangent=True)
+    try:
+        ns = 4 if setup.dim == 2 else 6
+        R = np.zeros((steps, ns, setup.dim))
+        dR = np.zeros((steps, ns, setup.dim, 3))
+        for t in range(steps):
+            o.step(1)
+            R[t], dR[t] = o.wall_tangent()
+        return R, dR
+    finally:
+        o.close()
 
 
 def segment_mass(dim, R, d):

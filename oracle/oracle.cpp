@@ -54,102 +54,111 @@
 #include <omp.h>
 #endif
 
+#include "dual.h"
 #include "perigrain.h"
 
 namespace {
 
 thread_local std::string g_err;
 
-template <unsigned D> struct Vec {
-  double c[D];
-  double &operator()(unsigned k) { return c[k]; }
-  double operator()(unsigned k) const { return c[k]; }
+// The scalar type T is double (the oracle) or CDual<N> (its tangent mode,
+// dual.h): same operations, same order, derivatives carried alongside.
+using std::asin;
+using std::fabs;
+using std::sqrt;
+
+template <unsigned D, class T> struct Vec {
+  T c[D];
+  T &operator()(unsigned k) { return c[k]; }
+  const T &operator()(unsigned k) const { return c[k]; }
   static Vec zero() {
     Vec v;
-    for (unsigned k = 0; k < D; ++k) v.c[k] = 0.0;
+    for (unsigned k = 0; k < D; ++k) v.c[k] = T(0.0);
     return v;
   }
 };
 
-template <unsigned D> Vec<D> operator+(const Vec<D> &a, const Vec<D> &b) {
-  Vec<D> r;
+template <unsigned D, class T> Vec<D, T> operator+(const Vec<D, T> &a, const Vec<D, T> &b) {
+  Vec<D, T> r;
   for (unsigned k = 0; k < D; ++k) r.c[k] = a.c[k] + b.c[k];
   return r;
 }
-template <unsigned D> Vec<D> operator-(const Vec<D> &a, const Vec<D> &b) {
-  Vec<D> r;
+template <unsigned D, class T> Vec<D, T> operator-(const Vec<D, T> &a, const Vec<D, T> &b) {
+  Vec<D, T> r;
   for (unsigned k = 0; k < D; ++k) r.c[k] = a.c[k] - b.c[k];
   return r;
 }
-template <unsigned D> Vec<D> operator*(double s, const Vec<D> &a) {
-  Vec<D> r;
+template <unsigned D, class T, class S> Vec<D, T> operator*(const S &s, const Vec<D, T> &a) {
+  Vec<D, T> r;
   for (unsigned k = 0; k < D; ++k) r.c[k] = s * a.c[k];
   return r;
 }
-template <unsigned D> Vec<D> operator/(const Vec<D> &a, double s) {
-  Vec<D> r;
+template <unsigned D, class T, class S> Vec<D, T> operator/(const Vec<D, T> &a, const S &s) {
+  Vec<D, T> r;
   for (unsigned k = 0; k < D; ++k) r.c[k] = a.c[k] / s;
   return r;
 }
-template <unsigned D> Vec<D> &operator+=(Vec<D> &a, const Vec<D> &b) {
+template <unsigned D, class T> Vec<D, T> &operator+=(Vec<D, T> &a, const Vec<D, T> &b) {
   for (unsigned k = 0; k < D; ++k) a.c[k] = a.c[k] + b.c[k];
   return a;
 }
 
 // Eigen 3.3.9 sum of a 2- or 3-term product expression (see header).
-inline double red(const Vec<2> &a, const Vec<2> &b) {
+template <class T> inline T red(const Vec<2, T> &a, const Vec<2, T> &b) {
   return a.c[0] * b.c[0] + a.c[1] * b.c[1];
 }
-inline double red(const Vec<3> &a, const Vec<3> &b) {
+template <class T> inline T red(const Vec<3, T> &a, const Vec<3, T> &b) {
   return (a.c[0] * b.c[0] + a.c[1] * b.c[1]) + a.c[2] * b.c[2];
 }
-template <unsigned D> inline double dot(const Vec<D> &a, const Vec<D> &b) {
+template <unsigned D, class T> inline T dot(const Vec<D, T> &a, const Vec<D, T> &b) {
   return red(a, b);
 }
-template <unsigned D> inline double norm(const Vec<D> &a) {
-  return std::sqrt(red(a, a));
+template <unsigned D, class T> inline T norm(const Vec<D, T> &a) {
+  return sqrt(red(a, a));
 }
 
 // ParticleN<dim> (particle2.h:28-86) plus slot bookkeeping used only to
 // report which original bond slots are still intact.
-template <unsigned D> struct Grain {
+template <unsigned D, class T> struct Grain {
+  using V = Vec<D, T>;
   unsigned nnodes = 0;
-  std::vector<Vec<D>> pos, disp, CurrPos, vel, acc, force, extforce;
+  std::vector<V> pos, disp, CurrPos, vel, acc, force, extforce;
   std::vector<double> vol;
   std::vector<unsigned> boundary_nodes, clamped_nodes;
   std::vector<std::vector<unsigned>> NbdArr;
   std::vector<unsigned> total_neighbors;
-  std::vector<std::vector<Vec<D>>> xi;
-  std::vector<std::vector<double>> xi_norm, stretch;
+  std::vector<std::vector<V>> xi;
+  std::vector<std::vector<T>> xi_norm, stretch;
   std::vector<std::vector<uint64_t>> slot; // original slot id of each entry
-  double delta = 0, rho = 0, cnot = 0, snot = 0;
+  double delta = 0, rho = 0, snot = 0;
+  T cnot = T(0.0);
   int breakable = 1, movable = 1, stoppable = 1;
   bool break_bonds = false;
   unsigned torque_axis = 2;
   double torque_val = 0;
 
   // mean_CurrPos (particle2.h:89-98)
-  Vec<D> mean_CurrPos() const {
-    Vec<D> mean = Vec<D>::zero();
+  V mean_CurrPos() const {
+    V mean = V::zero();
     for (unsigned i = 0; i < nnodes; i++) mean += (pos[i] + disp[i]);
     for (unsigned k = 0; k < D; ++k) mean.c[k] = mean.c[k] / (double)nnodes;
     return mean;
   }
 
   // get_peridynamic_force (particle2.h:102-188)
-  std::vector<Vec<D>> get_peridynamic_force() {
-    std::vector<Vec<D>> tot(nnodes);
+  std::vector<V> get_peridynamic_force() {
+    std::vector<V> tot(nnodes);
     for (unsigned i = 0; i < nnodes; ++i) {
-      Vec<D> v = Vec<D>::zero();
+      V v = V::zero();
       for (unsigned j = 0; j < total_neighbors[i]; ++j) {
         unsigned idx_nbr = NbdArr[i][j];
-        Vec<D> eta_p_xi = (disp[idx_nbr] - disp[i]) + xi[i][j];
-        double eta_p_xi_norm = norm(eta_p_xi);
-        Vec<D> unit_dir = eta_p_xi / eta_p_xi_norm;
-        double xi_norm_here = xi_norm[i][j];
-        double diff_norm = (eta_p_xi_norm - xi_norm_here);
-        double str = diff_norm / xi_norm_here;
-        Vec<D> v_more = ((cnot * str) * vol[idx_nbr]) * unit_dir;
+        V eta_p_xi = (disp[idx_nbr] - disp[i]) + xi[i][j];
+        T eta_p_xi_norm = norm(eta_p_xi);
+        V unit_dir = eta_p_xi / eta_p_xi_norm;
+        T xi_norm_here = xi_norm[i][j];
+        T diff_norm = (eta_p_xi_norm - xi_norm_here);
+        T str = diff_norm / xi_norm_here;
+        V v_more = ((cnot * str) * vol[idx_nbr]) * unit_dir;
         v += v_more;
         stretch[i][j] = str;
       }
@@ -205,25 +214,25 @@ template <unsigned D> struct Grain {
   }
 };
 
-struct ContactP { // Contact (contact.h:43-112)
-  double contact_rad, normal_stiffness, friction_coefficient, damping_ratio;
+template <class T> struct ContactP { // Contact (contact.h:43-112)
+  double contact_rad, damping_ratio;
+  T normal_stiffness, friction_coefficient;
   bool allow_friction, allow_damping, nl_bdry_only, self_contact;
 };
 
 // One reaction term: reaction[side](comp) -= value (contact.h:339,374,...).
-struct RTerm {
+template <class T> struct RTerm {
   uint8_t side, comp;
-  double value;
+  T value;
 };
 
 // segment_mass<dim> (contact.h:16-41): {centroid, area}.
-template <unsigned D> void segment_mass(double R, double d, double &centroid,
-                                        double &area) {
-  double Area, I;
+template <unsigned D, class T> void segment_mass(double R, T d, T &centroid, T &area) {
+  T Area, I;
   if (D == 2) {
-    Area = M_PI * (R * R) / 2 - (R * R) * std::asin(d / R) -
-           std::sqrt(R * R - d * d) * d;
-    I = 0.0 * std::pow((R * R - d * d), 1.5); // 2/3 == 0 (integer division)
+    Area = M_PI * (R * R) / 2 - (R * R) * asin(d / R) -
+           sqrt(R * R - d * d) * d;
+    I = T(0.0 * std::pow(sval(R * R - d * d), 1.5)); // 2/3 == 0 (integer division)
   } else {
     Area = M_PI / 3 * ((R - d) * (R - d)) * (2 * R - d);
     I = M_PI / 4 * ((R * R - d * d) * (R * R - d * d));
@@ -233,33 +242,33 @@ template <unsigned D> void segment_mass(double R, double d, double &centroid,
 }
 
 // wall_side_force_update (contact.h:310-431).  `side` indexes the reaction.
-template <unsigned D>
-void wall_side_force_update(double dist, const ContactP &CN, unsigned dir,
-                            const Vec<D> &vel_i, double vol_i, double rho,
-                            Vec<D> &f, double dt, double wall_side_speed,
-                            unsigned side, std::vector<RTerm> *terms) {
-  if (std::fabs(dist) <= CN.contact_rad) {
-    double c0, c1;
-    segment_mass<D>(CN.contact_rad, std::fabs(dist), c0, c1);
-    double contact_rad_contrib = (CN.contact_rad - c0);
-    double positive = (contact_rad_contrib > 0 ? contact_rad_contrib : 0);
-    double this_contact_force = (CN.normal_stiffness * positive * c1);
+template <unsigned D, class T>
+void wall_side_force_update(T dist, const ContactP<T> &CN, unsigned dir,
+                            const Vec<D, T> &vel_i, double vol_i, double rho,
+                            Vec<D, T> &f, double dt, double wall_side_speed,
+                            unsigned side, std::vector<RTerm<T>> *terms) {
+  if (fabs(dist) <= CN.contact_rad) {
+    T c0, c1;
+    segment_mass<D, T>(CN.contact_rad, fabs(dist), c0, c1);
+    T contact_rad_contrib = (CN.contact_rad - c0);
+    T positive = (contact_rad_contrib > 0 ? contact_rad_contrib : T(0.0));
+    T this_contact_force = (CN.normal_stiffness * positive * c1);
     double c_sign = (dist > 0 ? 1 : -1);
     f(dir) += (c_sign * this_contact_force);
     if (terms) terms->push_back({(uint8_t)side, (uint8_t)dir, vol_i * c_sign * this_contact_force});
 
-    Vec<D> rel_vel_i = vel_i;
+    Vec<D, T> rel_vel_i = vel_i;
     rel_vel_i(dir) -= wall_side_speed;
 
     if (CN.allow_friction) {
       if (D == 2) {
         unsigned dir_tang = (dir + 1) % 2;
-        if (std::fabs(rel_vel_i(dir_tang)) > 0) {
+        if (fabs(rel_vel_i(dir_tang)) > 0) {
           double sgn = (rel_vel_i(dir_tang) > 0 ? 1 : -1);
-          double friction_force =
-              (-CN.friction_coefficient) * std::fabs(this_contact_force) * sgn;
-          if (std::fabs(rel_vel_i(dir_tang)) <
-              CN.friction_coefficient * std::fabs(this_contact_force) * dt / rho) {
+          T friction_force =
+              (-CN.friction_coefficient) * fabs(this_contact_force) * sgn;
+          if (fabs(rel_vel_i(dir_tang)) <
+              CN.friction_coefficient * fabs(this_contact_force) * dt / rho) {
             friction_force = -rel_vel_i(dir_tang) * rho / dt;
           }
           f((dir + 1) % 2) += friction_force;
@@ -267,13 +276,13 @@ void wall_side_force_update(double dist, const ContactP &CN, unsigned dir,
         }
       } else {
         unsigned t1 = (dir + 1) % 3, t2 = (dir + 2) % 3;
-        double vp1 = rel_vel_i(t1), vp2 = rel_vel_i(t2);
-        double vpn = std::sqrt(vp1 * vp1 + vp2 * vp2);
+        T vp1 = rel_vel_i(t1), vp2 = rel_vel_i(t2);
+        T vpn = sqrt(vp1 * vp1 + vp2 * vp2);
         if (vpn > 0) {
-          double u1 = vp1 / vpn, u2 = vp2 / vpn;
-          double ff1 = (-CN.friction_coefficient) * std::fabs(this_contact_force) * u1;
-          double ff2 = (-CN.friction_coefficient) * std::fabs(this_contact_force) * u2;
-          if (vpn < CN.friction_coefficient * std::fabs(this_contact_force) * dt / rho) {
+          T u1 = vp1 / vpn, u2 = vp2 / vpn;
+          T ff1 = (-CN.friction_coefficient) * fabs(this_contact_force) * u1;
+          T ff2 = (-CN.friction_coefficient) * fabs(this_contact_force) * u2;
+          if (vpn < CN.friction_coefficient * fabs(this_contact_force) * dt / rho) {
             ff1 = -vpn / rho / dt * u1;
             ff2 = -vpn / rho / dt * u2;
           }
@@ -287,10 +296,10 @@ void wall_side_force_update(double dist, const ContactP &CN, unsigned dir,
       }
     }
     if (CN.allow_damping) {
-      double damping_coefficient = 2 * CN.damping_ratio *
-                                   std::sqrt(CN.normal_stiffness * rho) *
-                                   std::sqrt(std::fabs(c1));
-      double damping_force = (-damping_coefficient) * rel_vel_i(dir);
+      T damping_coefficient = 2 * CN.damping_ratio *
+                              sqrt(CN.normal_stiffness * rho) *
+                              sqrt(fabs(c1));
+      T damping_force = (-damping_coefficient) * rel_vel_i(dir);
       f(dir) += damping_force;
       if (terms) terms->push_back({(uint8_t)side, (uint8_t)dir, vol_i * damping_force});
     }
@@ -300,9 +309,9 @@ void wall_side_force_update(double dist, const ContactP &CN, unsigned dir,
 // update_contact_force_by_boundary (contact.h:667-812).  C and N are the same
 // object for self contact.  Pairs are appended to `pairs` (grain-local ids
 // offset by the given global offsets) when non-null.
-template <unsigned D>
-void contact_by_boundary(const Grain<D> &P_C, const Grain<D> &P_N,
-                         const ContactP &CN, double dt, std::vector<Vec<D>> &F,
+template <unsigned D, class T>
+void contact_by_boundary(const Grain<D, T> &P_C, const Grain<D, T> &P_N,
+                         const ContactP<T> &CN, double dt, std::vector<Vec<D, T>> &F,
                          bool is_self, std::vector<uint32_t> *pairs,
                          uint32_t off_C, uint32_t off_N) {
   unsigned tn_i = (CN.nl_bdry_only ? (unsigned)P_C.boundary_nodes.size() : P_C.nnodes);
@@ -311,11 +320,11 @@ void contact_by_boundary(const Grain<D> &P_C, const Grain<D> &P_N,
     unsigned i = (CN.nl_bdry_only ? P_C.boundary_nodes[p] : p);
     for (unsigned q = 0; q < tn_j; ++q) {
       unsigned j = (CN.nl_bdry_only ? P_N.boundary_nodes[q] : q);
-      Vec<D> dir = P_N.CurrPos[j] - P_C.CurrPos[i];
-      double dir_norm = norm(dir);
+      Vec<D, T> dir = P_N.CurrPos[j] - P_C.CurrPos[i];
+      T dir_norm = norm(dir);
       bool do_compute = true;
       double R_c = CN.contact_rad;
-      double K_n = CN.normal_stiffness;
+      T K_n = CN.normal_stiffness;
       if (is_self) {
         if (i == j) {
           do_compute = false;
@@ -324,7 +333,7 @@ void contact_by_boundary(const Grain<D> &P_C, const Grain<D> &P_N,
           if (std::find(row.begin(), row.end(), j) != row.end()) {
             do_compute = false;
           } else {
-            double ref_dist = norm(P_C.pos[i] - P_C.pos[j]);
+            double ref_dist = sval(norm(P_C.pos[i] - P_C.pos[j]));
             if (ref_dist < CN.contact_rad) {
               R_c = ref_dist;
               K_n = P_C.cnot / ref_dist;
@@ -340,29 +349,29 @@ void contact_by_boundary(const Grain<D> &P_C, const Grain<D> &P_N,
           pairs->push_back(off_C + i);
           pairs->push_back(off_N + j);
         }
-        Vec<D> unit_dir = dir / dir_norm;
-        double contact_rad_contrib = (R_c - dir_norm);
+        Vec<D, T> unit_dir = dir / dir_norm;
+        T contact_rad_contrib = (R_c - dir_norm);
         double N_vol = P_N.vol[j];
-        Vec<D> f_contact = ((-K_n * contact_rad_contrib) * N_vol) * unit_dir;
-        double contact_force_norm = norm(f_contact);
+        Vec<D, T> f_contact = ((-K_n * contact_rad_contrib) * N_vol) * unit_dir;
+        T contact_force_norm = norm(f_contact);
         F[i] += f_contact;
         if (CN.allow_damping) {
-          Vec<D> rel_v = P_C.vel[i] - P_N.vel[j];
-          double rel_v_proj = dot(rel_v, unit_dir);
-          Vec<D> rel_v_normal_comp = rel_v_proj * unit_dir;
-          double damping_coefficient = 2 * CN.damping_ratio *
-                                       std::sqrt(K_n * P_C.rho) *
-                                       std::sqrt(N_vol);
-          Vec<D> f_damping = (-damping_coefficient) * rel_v_normal_comp;
+          Vec<D, T> rel_v = P_C.vel[i] - P_N.vel[j];
+          T rel_v_proj = dot(rel_v, unit_dir);
+          Vec<D, T> rel_v_normal_comp = rel_v_proj * unit_dir;
+          T damping_coefficient = 2 * CN.damping_ratio *
+                                  sqrt(K_n * P_C.rho) *
+                                  sqrt(N_vol);
+          Vec<D, T> f_damping = (-damping_coefficient) * rel_v_normal_comp;
           F[i] += f_damping;
         }
         if (CN.allow_friction) {
-          Vec<D> rel_v = P_C.vel[i] - P_N.vel[j];
-          Vec<D> rel_v_tangential_comp = rel_v - dot(rel_v, unit_dir) * unit_dir;
-          double rel_v_tangential_norm = norm(rel_v_tangential_comp);
+          Vec<D, T> rel_v = P_C.vel[i] - P_N.vel[j];
+          Vec<D, T> rel_v_tangential_comp = rel_v - dot(rel_v, unit_dir) * unit_dir;
+          T rel_v_tangential_norm = norm(rel_v_tangential_comp);
           if (rel_v_tangential_norm > 0) {
-            Vec<D> unit_t = rel_v_tangential_comp / norm(rel_v_tangential_comp);
-            Vec<D> f_friction =
+            Vec<D, T> unit_t = rel_v_tangential_comp / norm(rel_v_tangential_comp);
+            Vec<D, T> f_friction =
                 (-CN.friction_coefficient * contact_force_norm) * unit_t;
             if (rel_v_tangential_norm <
                 CN.friction_coefficient * contact_force_norm * dt / P_C.rho) {
@@ -378,21 +387,21 @@ void contact_by_boundary(const Grain<D> &P_C, const Grain<D> &P_N,
 
 // get_minmax (contact.h:1023-1062): 2D [xmin,xmax,ymax,ymin],
 // 3D [xmin,ymin,zmin,xmax,ymax,zmax].
-template <unsigned D> void get_minmax(const std::vector<Vec<D>> &P, double *out) {
+template <unsigned D, class T> void get_minmax(const std::vector<Vec<D, T>> &P, double *out) {
   if (D == 2) {
-    out[0] = P[0](0); out[1] = P[0](0); out[2] = P[0](1); out[3] = P[0](1);
+    out[0] = sval(P[0](0)); out[1] = sval(P[0](0)); out[2] = sval(P[0](1)); out[3] = sval(P[0](1));
     for (size_t i = 0; i < P.size(); i++) {
-      out[0] = std::min(out[0], P[i](0));
-      out[1] = std::max(out[1], P[i](0));
-      out[2] = std::max(out[2], P[i](1));
-      out[3] = std::min(out[3], P[i](1));
+      out[0] = std::min(out[0], sval(P[i](0)));
+      out[1] = std::max(out[1], sval(P[i](0)));
+      out[2] = std::max(out[2], sval(P[i](1)));
+      out[3] = std::min(out[3], sval(P[i](1)));
     }
   } else {
-    for (unsigned k = 0; k < 3; ++k) { out[k] = P[0](k); out[3 + k] = P[0](k); }
+    for (unsigned k = 0; k < 3; ++k) { out[k] = sval(P[0](k)); out[3 + k] = sval(P[0](k)); }
     for (size_t i = 0; i < P.size(); i++) {
       for (unsigned k = 0; k < 3; ++k) {
-        out[k] = std::min(out[k], P[i](k));
-        out[3 + k] = std::max(out[3 + k], P[i](k));
+        out[k] = std::min(out[k], sval(P[i](k)));
+        out[3 + k] = std::max(out[3 + k], sval(P[i](k)));
       }
     }
   }
@@ -427,17 +436,34 @@ struct OracleBase {
   virtual int read_conn(int grain, uint32_t *dst, uint64_t cap, uint64_t *rows) = 0;
   virtual int read_contacts(uint32_t *dst, uint64_t cap, uint64_t *n) = 0;
   virtual int sizes(uint64_t *nn, uint64_t *ng, uint64_t *ns) = 0;
+  virtual int read_wall_tangent(double *reaction, double *dreaction) = 0;
 };
 
-template <unsigned D> struct Oracle : OracleBase {
-  std::vector<Grain<D>> PArr;
+// Seeds of the tangent mode (CDual<3>): d/d(common scale of every grain's
+// cnot, at 1), d/d(normal_stiffness), d/d(friction_coefficient).
+inline void seed_param(double v, double &x) { x = v; }
+template <int N> inline void seed_param(double v, CDual<N> &x) {
+  x = CDual<N>(v);
+}
+inline void seed_dir(double, int, double &) {}
+template <int N> inline void seed_dir(double scale, int dir, CDual<N> &x) {
+  if (dir < N) x.d[dir] = scale;
+}
+inline void deriv(const double &, double *, int) {}
+template <int N> inline void deriv(const CDual<N> &x, double *out, int stride) {
+  for (int k = 0; k < N; ++k) out[k * stride] = x.d[k];
+}
+
+template <unsigned D, class T> struct Oracle : OracleBase {
+  using V = Vec<D, T>;
+  std::vector<Grain<D, T>> PArr;
   std::vector<uint32_t> node_off;
-  ContactP CN;
+  ContactP<T> CN;
   pg_params TL;
   bool allow_wall = false;
   double wall[6] = {0, 0, 0, 0, 0, 0};
   double speed[6] = {0, 0, 0, 0, 0, 0};
-  double reaction[6][3];
+  T reaction[6][3];
   std::vector<uint8_t> intact;
   uint64_t n_slots = 0;
   uint32_t t_done = 0;
@@ -450,7 +476,7 @@ template <unsigned D> struct Oracle : OracleBase {
     node_off.resize(G + 1);
     uint64_t slot_base = 0;
     for (int g = 0; g < G; ++g) {
-      Grain<D> &P = PArr[g];
+      Grain<D, T> &P = PArr[g];
       int64_t a = s->node_offset[g], b = s->node_offset[g + 1];
       unsigned n = (unsigned)(b - a);
       node_off[g] = (uint32_t)a;
@@ -487,7 +513,9 @@ template <unsigned D> struct Oracle : OracleBase {
         for (size_t k = 0; k < P.NbdArr[i].size(); ++k) P.slot[i][k] = slot_base++;
       }
       P.gen_xi();
-      P.delta = s->delta[g]; P.rho = s->rho[g]; P.cnot = s->cnot[g]; P.snot = s->snot[g];
+      P.delta = s->delta[g]; P.rho = s->rho[g]; P.snot = s->snot[g];
+      seed_param(s->cnot[g], P.cnot);
+      seed_dir(s->cnot[g], 0, P.cnot);
       P.movable = s->movable[g]; P.breakable = s->breakable[g]; P.stoppable = s->stoppable[g];
       if (p->enable_torque) {
         P.torque_axis = s->torque_axis ? s->torque_axis[g] : 2;
@@ -499,8 +527,10 @@ template <unsigned D> struct Oracle : OracleBase {
     n_slots = slot_base;
     intact.assign(n_slots, 1);
     CN.contact_rad = s->contact_radius;
-    CN.normal_stiffness = s->normal_stiffness;
-    CN.friction_coefficient = s->friction_coefficient;
+    seed_param(s->normal_stiffness, CN.normal_stiffness);
+    seed_dir(1.0, 1, CN.normal_stiffness);
+    seed_param(s->friction_coefficient, CN.friction_coefficient);
+    seed_dir(1.0, 2, CN.friction_coefficient);
     CN.damping_ratio = s->damping_ratio;
     CN.allow_friction = p->allow_friction != 0;
     CN.allow_damping = p->allow_damping != 0;
@@ -508,14 +538,19 @@ template <unsigned D> struct Oracle : OracleBase {
     CN.self_contact = p->self_contact != 0;
     allow_wall = s->allow_wall != 0;
     for (int k = 0; k < 6; ++k) { wall[k] = s->wall[k]; speed[k] = p->wall_speed[k]; }
-    std::memset(reaction, 0, sizeof(reaction));
+    zero_reaction();
     return PG_OK;
+  }
+
+  void zero_reaction() {
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 3; ++b) reaction[a][b] = T(0.0);
   }
 
   // One iteration of the loop body at timeloop.h:608-878.
   void one_step(uint32_t t, int threads, bool rec_contacts) {
     const unsigned G = (unsigned)PArr.size();
-    std::memset(reaction, 0, sizeof(reaction)); // setzero_reaction
+    zero_reaction(); // setzero_reaction
     if (TL.set_movable_index != -1) {
       unsigned pi = (unsigned)TL.set_movable_index, ts = (unsigned)TL.set_movable_timestep;
       if (ts == t && pi < G) PArr[pi].movable = 1;
@@ -526,25 +561,25 @@ template <unsigned D> struct Oracle : OracleBase {
     }
     if (TL.reset_partzero_y && G > 1) {
       if (TL.reset_partzero_y_timestep == t) {
-        double max_bulk_y = PArr[1].pos[0](1) + PArr[1].disp[1](1);
+        double max_bulk_y = sval(PArr[1].pos[0](1) + PArr[1].disp[1](1));
         for (unsigned i = 1; i < G; ++i)
           for (unsigned j = 0; j < PArr[i].nnodes; j++) {
-            double now_y = PArr[i].pos[j](1) + PArr[i].disp[j](1);
+            double now_y = sval(PArr[i].pos[j](1) + PArr[i].disp[j](1));
             if (now_y > max_bulk_y) max_bulk_y = now_y;
           }
         double dest = max_bulk_y + TL.wheel_rad + CN.contact_rad;
-        double to_move_by = dest - PArr[0].mean_CurrPos()(1);
+        double to_move_by = dest - sval(PArr[0].mean_CurrPos()(1));
         for (unsigned j = 0; j < PArr[0].nnodes; j++) PArr[0].pos[j](1) += to_move_by;
       }
     }
     const double dt = TL.dt;
 #pragma omp parallel for num_threads(threads) if (threads > 1)
     for (unsigned i = 0; i < G; ++i) {
-      Grain<D> &P = PArr[i];
+      Grain<D, T> &P = PArr[i];
       if (P.movable) {
         const double h = (dt * dt * 0.5);
         for (unsigned n = 0; n < P.nnodes; ++n) {
-          Vec<D> inc = (dt * P.vel[n]) + (h * P.acc[n]);
+          V inc = (dt * P.vel[n]) + (h * P.acc[n]);
           P.disp[n] += inc;
         }
       }
@@ -554,13 +589,13 @@ template <unsigned D> struct Oracle : OracleBase {
     std::vector<double> bbox(G * 6);
     for (unsigned i = 0; i < G; ++i) get_minmax<D>(PArr[i].CurrPos, &bbox[i * 6]);
 
-    std::vector<std::vector<RTerm>> terms(G);
+    std::vector<std::vector<RTerm<T>>> terms(G);
     std::vector<std::vector<uint32_t>> gpairs(rec_contacts ? G : 0);
 
 #pragma omp parallel for schedule(dynamic) num_threads(threads) if (threads > 1)
     for (unsigned i = 0; i < G; ++i) {
-      Grain<D> &P = PArr[i];
-      std::vector<Vec<D>> ft(P.nnodes, Vec<D>::zero());
+      Grain<D, T> &P = PArr[i];
+      std::vector<V> ft(P.nnodes, V::zero());
       std::vector<uint32_t> *pairs = rec_contacts ? &gpairs[i] : nullptr;
       if (P.movable) {
         ft = P.get_peridynamic_force();
@@ -568,7 +603,7 @@ template <unsigned D> struct Oracle : OracleBase {
           if (P.breakable) {
             P.remove_bonds(intact);
             if (CN.self_contact)
-              contact_by_boundary<D>(P, P, CN, dt, ft, true, pairs, node_off[i], node_off[i]);
+              contact_by_boundary<D, T>(P, P, CN, dt, ft, true, pairs, node_off[i], node_off[i]);
           }
         }
         if (P.stoppable) {
@@ -576,7 +611,7 @@ template <unsigned D> struct Oracle : OracleBase {
           for (unsigned j = 0; j < G; j++) {
             if (j != i) {
               if (intersects_box_fext<D>(C, &bbox[j * 6], CN.contact_rad))
-                contact_by_boundary<D>(P, PArr[j], CN, dt, ft, false, pairs, node_off[i], node_off[j]);
+                contact_by_boundary<D, T>(P, PArr[j], CN, dt, ft, false, pairs, node_off[i], node_off[j]);
             }
           }
           if (allow_wall) {
@@ -592,20 +627,20 @@ template <unsigned D> struct Oracle : OracleBase {
           for (unsigned n = 0; n < P.nnodes; ++n) ft[n] += P.extforce[n];
         }
         if (TL.enable_torque) {
-          Vec<D> c = P.mean_CurrPos();
+          V c = P.mean_CurrPos();
           unsigned taxis = P.torque_axis;
           for (unsigned nn = 0; nn < P.nnodes; nn++) {
-            Vec<D> r = P.CurrPos[nn] - c;
-            Vec<D> rp = Vec<D>::zero();
-            if (D == 3) r((taxis) % D) = 0;
+            V r = P.CurrPos[nn] - c;
+            V rp = V::zero();
+            if (D == 3) r((taxis) % D) = T(0.0);
             rp((taxis + 1) % 3) = -r((taxis + 2) % 3);
             rp((taxis + 2) % 3) = r((taxis + 1) % 3);
-            double pn = norm(rp);
+            T pn = norm(rp);
             if (pn > 0) {
               rp = rp / pn;
             } else {
-              rp((taxis + 1) % 3) = 0;
-              rp((taxis + 2) % 3) = 0;
+              rp((taxis + 1) % 3) = T(0.0);
+              rp((taxis + 2) % 3) = T(0.0);
             }
             ft[nn] += (P.torque_val * rp);
           }
@@ -614,7 +649,7 @@ template <unsigned D> struct Oracle : OracleBase {
       P.force = ft;
     }
     for (unsigned i = 0; i < G; ++i)
-      for (const RTerm &r : terms[i]) reaction[r.side][r.comp] -= r.value;
+      for (const RTerm<T> &r : terms[i]) reaction[r.side][r.comp] -= r.value;
     if (rec_contacts) {
       contacts.clear();
       // pairs of grain i are appended in processing order; a stable sort by
@@ -631,10 +666,10 @@ template <unsigned D> struct Oracle : OracleBase {
 
 #pragma omp parallel for num_threads(threads) if (threads > 1)
     for (unsigned i = 0; i < G; ++i) {
-      Grain<D> &P = PArr[i];
+      Grain<D, T> &P = PArr[i];
       if (P.movable) {
         for (unsigned n = 0; n < P.nnodes; ++n) {
-          Vec<D> temp_acc = P.acc[n];
+          V temp_acc = P.acc[n];
           P.acc[n] = (1 / P.rho) * P.force[n];
           temp_acc += P.acc[n];
           for (unsigned k = 0; k < D; ++k) temp_acc(k) *= (0.5 * dt);
@@ -642,9 +677,9 @@ template <unsigned D> struct Oracle : OracleBase {
         }
         for (unsigned cn = 0; cn < P.clamped_nodes.size(); cn++) {
           unsigned cid = P.clamped_nodes[cn];
-          P.disp[cid] = Vec<D>::zero();
-          P.vel[cid] = Vec<D>::zero();
-          P.acc[cid] = Vec<D>::zero();
+          P.disp[cid] = V::zero();
+          P.vel[cid] = V::zero();
+          P.acc[cid] = V::zero();
         }
       }
     }
@@ -654,30 +689,30 @@ template <unsigned D> struct Oracle : OracleBase {
   }
 
   // update_wall_contact_force_by_boundary (contact.h:899-964)
-  void wall_force(const Grain<D> &P, double dt, std::vector<Vec<D>> &F,
-                  std::vector<RTerm> *terms) {
+  void wall_force(const Grain<D, T> &P, double dt, std::vector<V> &F,
+                  std::vector<RTerm<T>> *terms) {
     unsigned tn = (CN.nl_bdry_only ? (unsigned)P.boundary_nodes.size() : P.nnodes);
     for (unsigned p = 0; p < tn; p++) {
       unsigned i = (CN.nl_bdry_only ? P.boundary_nodes[p] : p);
-      const Vec<D> &x = P.CurrPos[i];
+      const V &x = P.CurrPos[i];
       if (D == 2) {
         // sides in lrtb() order: 0 left, 1 right, 2 top, 3 bottom
-        double yb = x(1) - wall[3], xl = x(0) - wall[0];
-        double yt = x(1) - wall[2], xr = x(0) - wall[1];
-        wall_side_force_update<D>(yb, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[3], 3, terms);
-        wall_side_force_update<D>(xl, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[0], 0, terms);
-        wall_side_force_update<D>(xr, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[1], 1, terms);
-        wall_side_force_update<D>(yt, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[2], 2, terms);
+        T yb = x(1) - wall[3], xl = x(0) - wall[0];
+        T yt = x(1) - wall[2], xr = x(0) - wall[1];
+        wall_side_force_update<D, T>(yb, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[3], 3, terms);
+        wall_side_force_update<D, T>(xl, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[0], 0, terms);
+        wall_side_force_update<D, T>(xr, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[1], 1, terms);
+        wall_side_force_update<D, T>(yt, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[2], 2, terms);
       } else {
         // sides: 0 x_min 1 y_min 2 z_min 3 x_max 4 y_max 5 z_max
-        double d0 = x(0) - wall[0], d1 = x(1) - wall[1], d2 = x(2) - wall[2];
-        double d3 = x(0) - wall[3], d4 = x(1) - wall[4], d5 = x(2) - wall[5];
-        wall_side_force_update<D>(d0, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[0], 0, terms);
-        wall_side_force_update<D>(d3, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[3], 3, terms);
-        wall_side_force_update<D>(d1, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[1], 1, terms);
-        wall_side_force_update<D>(d4, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[4], 4, terms);
-        wall_side_force_update<D>(d2, CN, 2, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[2], 2, terms);
-        wall_side_force_update<D>(d5, CN, 2, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[5], 5, terms);
+        T d0 = x(0) - wall[0], d1 = x(1) - wall[1], d2 = x(2) - wall[2];
+        T d3 = x(0) - wall[3], d4 = x(1) - wall[4], d5 = x(2) - wall[5];
+        wall_side_force_update<D, T>(d0, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[0], 0, terms);
+        wall_side_force_update<D, T>(d3, CN, 0, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[3], 3, terms);
+        wall_side_force_update<D, T>(d1, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[1], 1, terms);
+        wall_side_force_update<D, T>(d4, CN, 1, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[4], 4, terms);
+        wall_side_force_update<D, T>(d2, CN, 2, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[2], 2, terms);
+        wall_side_force_update<D, T>(d5, CN, 2, P.vel[i], P.vol[i], P.rho, F[i], dt, speed[5], 5, terms);
       }
     }
   }
@@ -699,8 +734,8 @@ template <unsigned D> struct Oracle : OracleBase {
 
   int read_nodes(int field, double *dst) override {
     for (size_t g = 0; g < PArr.size(); ++g) {
-      const Grain<D> &P = PArr[g];
-      const std::vector<Vec<D>> *src = nullptr;
+      const Grain<D, T> &P = PArr[g];
+      const std::vector<V> *src = nullptr;
       switch (field) {
         case PG_FIELD_CURRPOS: src = &P.CurrPos; break;
         case PG_FIELD_VEL: src = &P.vel; break;
@@ -710,7 +745,7 @@ template <unsigned D> struct Oracle : OracleBase {
         default: g_err = "oracle: bad field"; return PG_ERR_INVALID;
       }
       for (unsigned i = 0; i < P.nnodes; ++i)
-        for (unsigned k = 0; k < D; ++k) dst[(size_t)(node_off[g] + i) * D + k] = (*src)[i](k);
+        for (unsigned k = 0; k < D; ++k) dst[(size_t)(node_off[g] + i) * D + k] = sval((*src)[i](k));
     }
     return PG_OK;
   }
@@ -721,8 +756,23 @@ template <unsigned D> struct Oracle : OracleBase {
       // get_reaction() order equals the lrtb() side order used here
       int ns = (D == 2) ? 4 : 6;
       for (int s = 0; s < ns; ++s)
-        for (unsigned k = 0; k < D; ++k) react[s * D + k] = reaction[s][k];
+        for (unsigned k = 0; k < D; ++k) react[s * D + k] = sval(reaction[s][k]);
     }
+    return PG_OK;
+  }
+
+  // Reaction of the last step and its derivatives along the seeded
+  // parameters: dreaction[(s*D + k)*3 + q] (zero for the plain oracle).
+  int read_wall_tangent(double *react, double *dreact) override {
+    const int ns = (D == 2) ? 4 : 6;
+    for (int s = 0; s < ns; ++s)
+      for (unsigned k = 0; k < D; ++k) {
+        if (react) react[s * D + k] = sval(reaction[s][k]);
+        if (dreact) {
+          for (int q = 0; q < 3; ++q) dreact[(s * D + k) * 3 + q] = 0.0;
+          deriv(reaction[s][k], dreact + (s * D + k) * 3, 1);
+        }
+      }
     return PG_OK;
   }
 
@@ -734,7 +784,7 @@ template <unsigned D> struct Oracle : OracleBase {
   // NbdArr2conn (timeloop.h:25-49)
   int read_conn(int grain, uint32_t *dst, uint64_t cap, uint64_t *rows) override {
     if (grain < 0 || grain >= (int)PArr.size()) { g_err = "oracle: bad grain"; return PG_ERR_INVALID; }
-    const Grain<D> &P = PArr[grain];
+    const Grain<D, T> &P = PArr[grain];
     uint64_t r = 0;
     for (unsigned i = 0; i < P.nnodes; i++)
       for (unsigned q : P.NbdArr[i])
@@ -764,12 +814,12 @@ int oracle_create(const pg_setup *s, const pg_params *p, void **out) {
   try {
     if (!s || !p || !out) { g_err = "oracle: null argument"; return PG_ERR_INVALID; }
     if (s->dim == 2) {
-      auto *o = new Oracle<2>();
+      auto *o = new Oracle<2, double>();
       int rc = o->init(s, p);
       if (rc) { delete o; return rc; }
       *out = static_cast<OracleBase *>(o);
     } else if (s->dim == 3) {
-      auto *o = new Oracle<3>();
+      auto *o = new Oracle<3, double>();
       int rc = o->init(s, p);
       if (rc) { delete o; return rc; }
       *out = static_cast<OracleBase *>(o);
@@ -782,6 +832,36 @@ int oracle_create(const pg_setup *s, const pg_params *p, void **out) {
     g_err = e.what();
     return PG_ERR_ALLOC;
   }
+}
+
+// Tangent mode: the same restatement on CDual<3> numbers, seeded along
+// (cnot scale, normal_stiffness, friction_coefficient).
+int oracle_create_tangent(const pg_setup *s, const pg_params *p, void **out) {
+  try {
+    if (!s || !p || !out) { g_err = "oracle: null argument"; return PG_ERR_INVALID; }
+    if (s->dim == 2) {
+      auto *o = new Oracle<2, CDual<3>>();
+      int rc = o->init(s, p);
+      if (rc) { delete o; return rc; }
+      *out = static_cast<OracleBase *>(o);
+    } else if (s->dim == 3) {
+      auto *o = new Oracle<3, CDual<3>>();
+      int rc = o->init(s, p);
+      if (rc) { delete o; return rc; }
+      *out = static_cast<OracleBase *>(o);
+    } else {
+      g_err = "oracle: dim must be 2 or 3";
+      return PG_ERR_INVALID;
+    }
+    return PG_OK;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return PG_ERR_ALLOC;
+  }
+}
+
+int oracle_read_wall_tangent(void *h, double *reaction, double *dreaction) {
+  return static_cast<OracleBase *>(h)->read_wall_tangent(reaction, dreaction);
 }
 
 int oracle_destroy(void *h) {
@@ -820,8 +900,8 @@ int oracle_max_threads(void) {
 
 // Scalar kernels exposed for known-answer tests.
 void oracle_segment_mass(int dim, double R, double d, double *centroid, double *area) {
-  if (dim == 2) segment_mass<2>(R, d, *centroid, *area);
-  else segment_mass<3>(R, d, *centroid, *area);
+  if (dim == 2) segment_mass<2, double>(R, d, *centroid, *area);
+  else segment_mass<3, double>(R, d, *centroid, *area);
 }
 
 } // extern "C"

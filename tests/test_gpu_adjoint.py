@@ -6,7 +6,7 @@ import pytest
 
 torch = pytest.importorskip("torch")  # before libpg: one HIP runtime for both
 
-from perigrain_amd import adjoint, setups  # noqa: E402
+from perigrain_amd import adjoint, diffrun  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -17,7 +17,7 @@ THETA_T = (1.0e9, 1.2, 0.4)
 def _small_run(steps=240, ck=50, **kw):
     kw.setdefault("n_grains", 4)
     kw.setdefault("nodes_per_grain", 60)
-    return adjoint.compression_2d_run(steps=steps, checkpoint_every=ck, **kw)
+    return diffrun.compression_2d_run(steps=steps, checkpoint_every=ck, **kw)
 
 
 def test_recorded_history_equals_plain_steps():
@@ -74,3 +74,22 @@ def test_gradient_matches_central_differences(ck):
     print("adjoint", grad, "central differences", fd, flush=True)
     rel = np.abs(grad - fd) / np.abs(fd)
     assert np.all(rel < 1e-5), f"adjoint {grad} vs central differences {fd}: rel {rel}"
+
+
+def test_gradient_matches_cpu_dual_number_reference(oracle_threads):
+    # The FP64 CPU reference: the oracle on dual numbers, seeded along the
+    # same three parameters, differentiates the same frozen-branch map.
+    import oracle
+    run = _small_run()
+    target = run.plain(*THETA_T)
+    hist, state = run.forward(*THETA0)
+    _, bar = diffrun.top_wall_misfit(hist, target)
+    _, raw = run.backward(state, bar)
+    s, p = run.make_setup(*THETA0)
+    R, dR = oracle.reaction_history_tangent(s, p, run.steps, threads=oracle_threads)
+    np.testing.assert_allclose(hist, R, rtol=1e-10, atol=1e-10 * np.abs(R).max())
+    bar_c = np.zeros_like(R)
+    bar_c[:, 2] = 2.0 * (R[:, 2] - target[:, 2])
+    raw_cpu = np.einsum("tsk,tskq->q", bar_c, dR)
+    print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
+    np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)

@@ -445,3 +445,42 @@ def test_generated_bonds_are_within_horizon_and_unique():
         # every pair within delta is bonded (convex disk)
         from scipy.spatial import cKDTree
         assert len(cKDTree(x).query_pairs(s.delta[gi])) == len(c)
+
+
+def _raw_loss(s, p, steps, target, threads=1):
+    o = oracle.Oracle(s, p, threads=threads)
+    L = 0.0
+    for t in range(steps):
+        o.step(1)
+        L += float(((o.wall()[1][2] - target[t, 2]) ** 2).sum())
+    o.close()
+    return L
+
+
+def test_tangent_mode_matches_central_differences():
+    # The oracle on dual numbers differentiates the wall-reaction history
+    # along (cnot scale, normal stiffness, friction); a misfit's gradient
+    # must match central differences of the plain oracle where the map is
+    # smooth (no branch switches in the small interval).
+    from dataclasses import replace
+    s, p = setups.compression_2d_diff(n_grains=4, nodes_per_grain=40)
+    steps = 80
+    R, dR = oracle.reaction_history_tangent(s, p, steps)
+    target = 0.9 * R
+    bar = np.zeros_like(R)
+    bar[:, 2] = 2.0 * (R[:, 2] - target[:, 2])
+    grad = np.einsum("tsk,tskq->q", bar, dR)
+    base = (1.0, s.normal_stiffness, s.friction_coefficient)
+    for q in range(3):
+        h = 1e-6 * base[q]
+        vals = []
+        for sg in (1, -1):
+            if q == 0:
+                sq = replace(s, cnot=s.cnot * (1.0 + sg * h))
+            elif q == 1:
+                sq = replace(s, normal_stiffness=s.normal_stiffness + sg * h)
+            else:
+                sq = replace(s, friction_coefficient=s.friction_coefficient + sg * h)
+            vals.append(_raw_loss(sq, p, steps, target))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(grad[q] - fd) <= 1e-5 * abs(fd), (q, grad[q], fd)
