@@ -139,6 +139,10 @@ struct pg_handle {
     unsigned long long *fin_box = nullptr;
     size_t cap_ck = 0, cap_seg = 0, cap_T = 0;
   } adj;
+  // LDS windows of the step kernel (k_step_lds); off when a window would not fit
+  bool lds_on = false;
+  uint32_t lds_slices = 0, lds_blocks = 0, lds_threads = 0, lds_bytes = 0;
+  uint32_t *d_win_lo = nullptr, *d_win_cnt = nullptr;
   // host copy of the bond table, valid for step bcol_host_t
   std::vector<uint32_t> bcol_host;
   uint32_t bcol_host_t = 0xFFFFFFFFu;
@@ -226,6 +230,12 @@ void fill_args(pg_handle *h) {
   A.any_torque = h->any_torque ? 1 : 0;
   A.uniform_vol = h->uniform_vol ? 1 : 0;
   A.break_step = h->adj.break_step;
+  A.win_lo = h->lds_on ? h->d_win_lo : nullptr;
+  A.win_cnt = h->lds_on ? h->d_win_cnt : nullptr;
+  A.blk_slices = h->lds_slices;
+  A.lds_blocks = h->lds_blocks;
+  A.lds_threads = h->lds_threads;
+  A.lds_bytes = h->lds_bytes;
 }
 
 int scan(pg_handle *h, const uint32_t *in, uint32_t *out, size_t n) {
@@ -670,6 +680,53 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
 
   // ---- device allocations
   HIPCHK(hipSetDevice(device));
+
+  // ---- LDS windows: blocks of S consecutive slices (S chosen so that the
+  // blocks just cover the CUs once when they can), each with the node range
+  // holding all bond partners of its nodes
+  std::vector<uint32_t> win_lo, win_cnt;
+  {
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (cus < 1) cus = 1;
+    const char *env = std::getenv("PG_LDS");
+    const bool want = !(env && env[0] == '0');
+    constexpr uint32_t kLdsLimit = 160u * 1024u; // gfx950: 160 KiB per CU
+    const uint32_t rec_bytes = 2u * D * 8u;
+    uint32_t spb = std::max<uint32_t>(1, (h->S + (uint32_t)cus - 1) / (uint32_t)cus);
+    spb = std::min<uint32_t>(spb, 32);
+    for (; want && spb >= 1; spb = (spb > 1) ? (spb + 1) / 2 : 0) {
+      const uint32_t nb = (h->S + spb - 1) / spb;
+      win_lo.assign(nb, 0);
+      win_cnt.assign(nb, 0);
+      uint64_t worst = 0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        const uint64_t i0 = (uint64_t)b * spb * SLICE, i1 = std::min<uint64_t>(N, (uint64_t)(b + 1) * spb * SLICE);
+        uint64_t lo = i0, hi = i1 > i0 ? i1 - 1 : i0;
+        for (uint32_t sl = b * spb; sl < std::min<uint32_t>(h->S, (b + 1) * spb); ++sl)
+          for (uint64_t e = h->slice_base[sl]; e < (uint64_t)h->slice_base[sl] + (uint64_t)h->slice_width[sl] * SLICE; ++e) {
+            const uint32_t w = bcol_init[e];
+            if (w == BCOL_PAD) continue;
+            const uint64_t j = w & BCOL_INDEX;
+            lo = std::min(lo, j);
+            hi = std::max(hi, j);
+          }
+        win_lo[b] = (uint32_t)lo;
+        win_cnt[b] = (uint32_t)(hi - lo + 1);
+        worst = std::max<uint64_t>(worst, (uint64_t)(hi - lo + 1) * rec_bytes);
+      }
+      if (worst <= kLdsLimit) {
+        h->lds_on = true;
+        h->lds_slices = spb;
+        h->lds_blocks = nb;
+        h->lds_threads = std::min<uint32_t>(spb, 16) * SLICE;
+        h->lds_bytes = (uint32_t)worst;
+        break;
+      }
+      if (spb == 1) break;
+    }
+    if (h->lds_on) HIPCHK(set_step_lds_limit(D, h->lds_bytes));
+  }
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
   for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&h->kev[k]));
@@ -694,6 +751,10 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->d_slice_base, S));
   RC(dalloc(h, h->d_slice_width, S));
   RC(dalloc(h, h->d_bcol, E + BCOL_TAIL));
+  if (h->lds_on) {
+    RC(dalloc(h, h->d_win_lo, win_lo.size()));
+    RC(dalloc(h, h->d_win_cnt, win_cnt.size()));
+  }
   RC(dalloc(h, h->cl_ptr, (size_t)N + 1));
   RC(dalloc(h, h->cl_cnt, (size_t)N + 1));
   h->cl_cap = 1024;
@@ -751,6 +812,10 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(h2d(h, h->d_slice_base, h->slice_base.data(), S));
   RC(h2d(h, h->d_slice_width, h->slice_width.data(), S));
   RC(h2d(h, h->d_bcol, bcol_init.data(), bcol_init.size()));
+  if (h->lds_on) {
+    RC(h2d(h, h->d_win_lo, win_lo.data(), win_lo.size()));
+    RC(h2d(h, h->d_win_cnt, win_cnt.size() ? win_cnt.data() : nullptr, win_cnt.size()));
+  }
   std::vector<double> walls((size_t)n * 6);
   for (int s = 0; s < n; ++s)
     for (int k = 0; k < 6; ++k) walls[(size_t)s * 6 + k] = setups[s].wall[k];

@@ -78,6 +78,12 @@ __device__ __forceinline__ bool fast_num_ok(double a) {
   return a == 0.0 || (m >= 0x1p-300 && m <= 0x1p300);
 }
 __device__ __forceinline__ bool fast_sqrt_ok(double x) { return x >= 0x1p-600 && x <= 0x1p600; }
+// x in [2^-600, 2^601) for x >= 0, from the high word: biased exponent in
+// [423, 1623] (zero, NaN and infinities fail)
+__device__ __forceinline__ bool sqrt_arg_in_range(double x) {
+  const uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(x) >> 32);
+  return hi - 0x1A700000u < 0x4B100000u;
+}
 // a/b through a precomputed reciprocal; a zero numerator returns itself
 // (the IEEE quotient of +-0 by a positive number).
 __device__ __forceinline__ double div_r(double a, double b, double r) {

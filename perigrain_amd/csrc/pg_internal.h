@@ -131,6 +131,12 @@ struct StepArgs {
   uint32_t *violation;         // first step whose positions broke the skin
   const uint32_t *tbase;       // graph replays: base step added to the launch's step argument
   uint32_t *break_step;        // differentiable runs: step at which each slot's bond broke
+  // LDS windows (k_step_lds): block b covers blk_slices slices and reads the
+  // records of nodes win_lo[b] .. win_lo[b] + win_cnt[b] - 1
+  const uint32_t *win_lo;
+  const uint32_t *win_cnt;
+  uint32_t blk_slices;
+  uint32_t lds_blocks, lds_threads, lds_bytes;
   double *force_out;           // [dim][ns], record steps
   double *wterm;               // record steps: [ckeys][6][4]
   uint8_t *wflag;              // record steps: [ckeys]

@@ -12,6 +12,7 @@ namespace pg {
 
 hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int block,
                        hipStream_t s);
+hipError_t set_step_lds_limit(int dim, uint32_t bytes);
 hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s);
 hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gmean,
                              hipStream_t s);

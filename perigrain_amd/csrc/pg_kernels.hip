@@ -85,14 +85,22 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
 #if PG_FAST_DIVSQRT
   // |xi| = sqrt(xin2), |eta| = sqrt(n2), s = (n - |xi|)/|xi|, e = eta/n,
   // through the shortened sequences (one reciprocal for the three e_q).
+  // Exact when xin2, n2 lie in [2^-600, 2^600] (so n, |xi| lie in
+  // [2^-300, 2^300]) and each numerator is zero or of magnitude >= 2^-300
+  // (their magnitudes are at most n or max(n, |xi|)).  A zero numerator
+  // gives a zero quotient whose sign may differ from the IEEE one; a zero of
+  // either sign adds nothing to F and compares equal in the break test, so
+  // the results are the same.
   double xin = sqrt_nr(xin2), n = sqrt_nr(n2);
-  double s = div_r(n - xin, xin, recip_nr(xin));
+  const double dn = n - xin;
+  double s = div_nr(dn, xin, recip_nr(xin));
   const double rn = recip_nr(n);
 #pragma unroll
-  for (int q = 0; q < D; ++q) e[q] = div_r(eta[q], n, rn);
-  bool ok = fast_sqrt_ok(xin2) && fast_sqrt_ok(n2) && fast_num_ok(n - xin);
+  for (int q = 0; q < D; ++q) e[q] = div_nr(eta[q], n, rn);
+  int emin = __builtin_amdgcn_frexp_exp(dn); // zero -> 0, |x| >= 2^-300 -> >= -299
 #pragma unroll
-  for (int q = 0; q < D; ++q) ok = ok && fast_num_ok(eta[q]);
+  for (int q = 0; q < D; ++q) emin = min(emin, __builtin_amdgcn_frexp_exp(eta[q]));
+  const bool ok = sqrt_arg_in_range(xin2) && sqrt_arg_in_range(n2) && emin >= -299;
   if (live && !ok) { // outside the fast range: IEEE operators
     xin = sqrt(xin2);
     n = sqrt(n2);
@@ -221,27 +229,63 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 // grain bounding boxes of step t+1 (get_minmax, contact.h:1023-1062) and
 // the Verlet-skin check.  Reads u^t from u[t&1] and v^{t-1} from
 // vel[(t+1)&1]; writes u^{t+1} to u[(t+1)&1] and v^t to vel[t&1].
-template <int D, bool RECORD, bool UVOL>
-__global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
-                                                                   const uint32_t t_arg) {
-  const uint32_t t = step_index(A, t_arg);
-  if (*A.violation <= t) return; // neighbour list stale since an earlier step
-  const uint32_t nb = gridDim.x;
-  const uint32_t lb = xcd_block(blockIdx.x, nb);
-  const uint32_t i = lb * blockDim.x + threadIdx.x;
-  const uint32_t nthreads = nb * blockDim.x;
-  const uint32_t ns = A.ns;
-
-  { // housekeeping: clear the box buffer of step t+2, move the walls
-    unsigned long long *bz = A.bbox[(t + 2) % 3];
-    const uint32_t nz = A.n_grains * 2 * D;
-    for (uint32_t q = i; q < nz; q += nthreads) bz[q] = ((q % (2 * D)) < D) ? ~0ull : 0ull;
-    if (i < A.n_insts * 6) {
-      const InstDev &ip = A.insts[i / 6];
-      A.wallpos[(t + 1) & 1][i] = A.wallpos[t & 1][i] + ip.speed[i % 6] * ip.dt;
-    }
+// Where the bond loop reads partner records from: the global record array
+// of step t, or a workgroup's window of it staged in LDS.
+template <int D, bool UVOL> struct GlobalRecs {
+  const double *U;
+  const double *V;
+  __device__ __forceinline__ void load(uint32_t cw, uint32_t self, double gvol, Nbr<D> &nb) const {
+    load_nbr<D, UVOL>(U, V, gvol, cw, self, nb);
   }
+};
+template <int D, bool UVOL> struct LdsRecs {
+  const double *W; // LDS: records of nodes lo .. lo+cnt-1
+  uint32_t lo;
+  const double *V;
+  // Broken and padding slots read the window's first record: their term is
+  // computed and discarded (bond_term adds +0.0 for them).
+  __device__ __forceinline__ void load(uint32_t cw, uint32_t, double gvol, Nbr<D> &nb) const {
+    const uint32_t off = (cw & BCOL_BROKEN) ? 0u : cw - lo;
+    const double2 *p = reinterpret_cast<const double2 *>(W + off * (2 * D));
+    if constexpr (D == 3) {
+      const double2 a = p[0], b = p[1], c = p[2];
+      nb.X[0] = a.x;
+      nb.X[1] = a.y;
+      nb.X[2] = b.x;
+      nb.u[0] = b.y;
+      nb.u[1] = c.x;
+      nb.u[2] = c.y;
+    } else {
+      const double2 a = p[0], b = p[1];
+      nb.X[0] = a.x;
+      nb.X[1] = a.y;
+      nb.u[0] = b.x;
+      nb.u[1] = b.y;
+    }
+    nb.V = UVOL ? gvol : V[(cw & BCOL_BROKEN) ? lo : cw];
+  }
+};
 
+// Per-launch housekeeping, spread over all threads of the grid: clear the box
+// buffer of step t+2 and move the walls.
+template <int D>
+__device__ __forceinline__ void step_housekeeping(const StepArgs &A, uint32_t t, uint32_t gi,
+                                                  uint32_t nthreads) {
+  unsigned long long *bz = A.bbox[(t + 2) % 3];
+  const uint32_t nz = A.n_grains * 2 * D;
+  for (uint32_t q = gi; q < nz; q += nthreads) bz[q] = ((q % (2 * D)) < D) ? ~0ull : 0ull;
+  if (gi < A.n_insts * 6) {
+    const InstDev &ip = A.insts[gi / 6];
+    A.wallpos[(t + 1) & 1][gi] = A.wallpos[t & 1][gi] + ip.speed[gi % 6] * ip.dt;
+  }
+}
+
+// The step of node i (all lanes of a wavefront call it together: the grain
+// boxes are combined across the wavefront).
+template <int D, bool RECORD, bool UVOL, class RS>
+__device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, const uint32_t i,
+                                          const RS &rs) {
+  const uint32_t ns = A.ns;
   const bool valid = i < A.n_nodes;
   uint32_t g = 0xFFFFFFFFu;
   double xn[D];
@@ -287,20 +331,20 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
         Nbr<D> ra0, ra1, rb0, rb1;
         uint4 cg = colg(0);
         uint4 cn = colg(1);
-        load_nbr<D, UVOL>(U, A.V, gvol, cg.x, i, ra0);
-        load_nbr<D, UVOL>(U, A.V, gvol, cg.y, i, ra1);
+        rs.load(cg.x, i, gvol, ra0);
+        rs.load(cg.y, i, gvol, ra1);
         // Issue order matters: loads complete in order, so the next column
         // group (no dependences) goes out before the record loads that need
         // the group loaded one iteration earlier.
         for (uint32_t q = 0; BGROUP * q < width; ++q) {
           uint32_t *sq = bc + (SLICE * BGROUP) * q;
-          load_nbr<D, UVOL>(U, A.V, gvol, cg.z, i, rb0);
-          load_nbr<D, UVOL>(U, A.V, gvol, cg.w, i, rb1);
+          rs.load(cg.z, i, gvol, rb0);
+          rs.load(cg.w, i, gvol, rb1);
           bond_term<D>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F, A, t);
           bond_term<D>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F, A, t);
           const uint4 cnn = colg(q + 2);
-          load_nbr<D, UVOL>(U, A.V, gvol, cn.x, i, ra0);
-          load_nbr<D, UVOL>(U, A.V, gvol, cn.y, i, ra1);
+          rs.load(cn.x, i, gvol, ra0);
+          rs.load(cn.y, i, gvol, ra1);
           bond_term<D>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F, A, t);
           bond_term<D>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F, A, t);
           cg = cn;
@@ -464,6 +508,45 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
 
   if (viol) atomicMin(A.violation, t + 1);
   wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
+}
+
+template <int D, bool RECORD, bool UVOL>
+__global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
+                                                                   const uint32_t t_arg) {
+  const uint32_t t = step_index(A, t_arg);
+  if (*A.violation <= t) return; // neighbour list stale since an earlier step
+  const uint32_t nb = gridDim.x;
+  const uint32_t lb = xcd_block(blockIdx.x, nb);
+  const uint32_t i = lb * blockDim.x + threadIdx.x;
+  step_housekeeping<D>(A, t, i, nb * blockDim.x);
+  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>{A.rec[t & 1], A.V});
+}
+
+// The same step with each workgroup's partner window staged in LDS: block b
+// owns A.blk_slices consecutive 64-node slices, and every bond partner of its
+// nodes lies in [win_lo[b], win_lo[b] + win_cnt[b]), which it copies from the
+// record array of step t with coalesced loads before the bond loops read it.
+template <int D, bool RECORD, bool UVOL>
+__global__ __launch_bounds__(1024) void k_step_lds(const StepArgs A, const uint32_t t_arg) {
+  extern __shared__ double lds_rec[];
+  const uint32_t t = step_index(A, t_arg);
+  if (*A.violation <= t) return; // uniform: no launch writes a value <= t
+  const uint32_t nb = gridDim.x;
+  const uint32_t lb = xcd_block(blockIdx.x, nb);
+  step_housekeeping<D>(A, t, lb * blockDim.x + threadIdx.x, nb * blockDim.x);
+  const uint32_t lo = A.win_lo[lb], cnt = A.win_cnt[lb];
+  {
+    const double2 *src = reinterpret_cast<const double2 *>(A.rec[t & 1] + (size_t)lo * 2 * D);
+    double2 *dst = reinterpret_cast<double2 *>(lds_rec);
+    const uint32_t n2 = cnt * D; // records are D double2 each
+    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x) dst[q] = src[q];
+  }
+  __syncthreads();
+  const uint32_t S = A.blk_slices;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const LdsRecs<D, UVOL> rs{lds_rec, lo, A.V};
+  for (uint32_t sl = wave; sl < S; sl += nw)
+    step_node<D, RECORD, UVOL>(A, t, (lb * S + sl) * 64 + lane, rs);
 }
 
 // Position update of step 1 from the initial state (timeloop.h:661-669),
@@ -816,8 +899,29 @@ static inline uint32_t nblk(size_t n, int b) { return (uint32_t)((n + b - 1) / b
 template <int D, bool RECORD>
 static void launch_step_t(bool uvol, const StepArgs &A, uint32_t t, uint32_t nb, int block,
                           hipStream_t s) {
+  if (A.win_lo) {
+    const dim3 g(A.lds_blocks), b(A.lds_threads);
+    if (uvol) hipLaunchKernelGGL((k_step_lds<D, RECORD, true>), g, b, A.lds_bytes, s, A, t);
+    else hipLaunchKernelGGL((k_step_lds<D, RECORD, false>), g, b, A.lds_bytes, s, A, t);
+    return;
+  }
   if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true>), dim3(nb), dim3(block), 0, s, A, t);
   else hipLaunchKernelGGL((k_step<D, RECORD, false>), dim3(nb), dim3(block), 0, s, A, t);
+}
+
+template <int D>
+static hipError_t lds_attr_t(uint32_t bytes) {
+  hipError_t e = hipSuccess;
+  const void *f[4] = {(const void *)k_step_lds<D, false, false>, (const void *)k_step_lds<D, false, true>,
+                      (const void *)k_step_lds<D, true, false>, (const void *)k_step_lds<D, true, true>};
+  for (int q = 0; q < 4 && e == hipSuccess; ++q)
+    e = hipFuncSetAttribute(f[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return e;
+}
+
+// Allow the LDS step kernels `bytes` of dynamic LDS per workgroup.
+hipError_t set_step_lds_limit(int dim, uint32_t bytes) {
+  return dim == 2 ? lds_attr_t<2>(bytes) : lds_attr_t<3>(bytes);
 }
 
 hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int block,

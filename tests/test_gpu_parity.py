@@ -99,7 +99,11 @@ def test_two_disk_fracture_bit_exact(oracle_threads):
     assert (o.bond_intact() == 0).sum() > 0, "workload should break bonds"
 
 
-def test_3d_compression_with_walls_bit_exact(oracle_threads):
+@pytest.mark.parametrize("lds", ["1", "0"])
+def test_3d_compression_with_walls_bit_exact(oracle_threads, monkeypatch, lds):
+    # lds "0" runs the step kernel that gathers partner records from global
+    # memory instead of from a workgroup's LDS window
+    monkeypatch.setenv("PG_LDS", lds)
     s, p = setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-5.0)
     g, o = _both(s, p, oracle_threads)
     done = 0
