@@ -689,11 +689,17 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (cus < 1) cus = 1;
+    // PG_LDS=1 forces the LDS kernel, PG_LDS=0 the gathering one.  Unset,
+    // the LDS kernel is used for small handles (at most one 64-node slice per
+    // CU), where it measured faster; on larger ones the step is VALU-bound
+    // and the gathering kernel, with finer-grained scheduling, is faster.
     const char *env = std::getenv("PG_LDS");
-    const bool want = !(env && env[0] == '0');
+    const bool force = env && env[0] == '1';
+    const bool want = force || (!(env && env[0] == '0') && h->S <= (uint32_t)cus);
     constexpr uint32_t kLdsLimit = 160u * 1024u; // gfx950: 160 KiB per CU
     const uint32_t rec_bytes = 2u * D * 8u;
     uint32_t spb = std::max<uint32_t>(1, (h->S + (uint32_t)cus - 1) / (uint32_t)cus);
+    if (!force && spb > 16) spb = 16; // one slice per wavefront; more blocks
     spb = std::min<uint32_t>(spb, 32);
     for (; want && spb >= 1; spb = (spb > 1) ? (spb + 1) / 2 : 0) {
       const uint32_t nb = (h->S + spb - 1) / spb;
@@ -715,7 +721,14 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         win_cnt[b] = (uint32_t)(hi - lo + 1);
         worst = std::max<uint64_t>(worst, (uint64_t)(hi - lo + 1) * rec_bytes);
       }
-      if (worst <= kLdsLimit) {
+      // blocks resident per CU (LDS and 16 wavefronts of 128 VGPRs), rounds
+      const uint32_t waves = std::min<uint32_t>(spb, 16);
+      const uint64_t per_cu = std::max<uint64_t>(
+          1, std::min<uint64_t>(worst ? kLdsLimit / worst : 16, 16 / waves));
+      const double rounds = (double)nb / ((double)cus * (double)per_cu);
+      const double waste = rounds > 1.0 ? std::ceil(rounds) / rounds : 1.0;
+      const bool balanced = waste <= 1.10 && spb <= 16;
+      if (worst <= kLdsLimit && (force || balanced)) {
         h->lds_on = true;
         h->lds_slices = spb;
         h->lds_blocks = nb;
@@ -723,7 +736,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         h->lds_bytes = (uint32_t)worst;
         break;
       }
-      if (spb == 1) break;
+      if (spb == 1 || (worst <= kLdsLimit && !balanced)) break;
     }
     if (h->lds_on) HIPCHK(set_step_lds_limit(D, h->lds_bytes));
   }
