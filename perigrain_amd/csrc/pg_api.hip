@@ -229,6 +229,7 @@ void fill_args(pg_handle *h) {
   A.n_insts = h->I;
   A.any_torque = h->any_torque ? 1 : 0;
   A.uniform_vol = h->uniform_vol ? 1 : 0;
+  A.ghost = h->N;
   A.break_step = h->adj.break_step;
   A.win_lo = h->lds_on ? h->d_win_lo : nullptr;
   A.win_cnt = h->lds_on ? h->d_win_cnt : nullptr;
@@ -719,7 +720,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
           }
         win_lo[b] = (uint32_t)lo;
         win_cnt[b] = (uint32_t)(hi - lo + 1);
-        worst = std::max<uint64_t>(worst, (uint64_t)(hi - lo + 1) * rec_bytes);
+        worst = std::max<uint64_t>(worst, (uint64_t)(hi - lo + 2) * rec_bytes); // + ghost
       }
       // blocks resident per CU (LDS and 16 wavefronts of 128 VGPRs), rounds
       const uint32_t waves = std::min<uint32_t>(spb, 16);
@@ -743,7 +744,9 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&h->ev[k]));
   for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&h->kev[k]));
-  h->ns = (uint32_t)(((N + SLICE - 1) / SLICE) * SLICE);
+  // one spare record at index N at least: the ghost partner of broken and
+  // padding bond slots
+  h->ns = (uint32_t)(((N + 1 + SLICE - 1) / SLICE) * SLICE);
   if (h->ns == 0) h->ns = SLICE;
   const size_t ND = (size_t)h->ns * D; // component-major [D][ns]
   RC(dalloc(h, h->rec[0], 2 * ND));
@@ -797,6 +800,9 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     const uint32_t nsd = h->ns;
     std::vector<double> sR(2 * ND, 0.0), sV(nsd, 0.0), sVel(ND, 0.0), sAcc(ND, 0.0),
         sExt(ND, 0.0);
+    // the ghost: far from every node (1e30 in each coordinate), volume 1
+    for (int k = 0; k < D; ++k) sR[N * 2 * D + k] = 1e30;
+    sV[N] = 1.0;
     for (uint64_t i = 0; i < N; ++i) {
       sV[i] = rec[i].X[D];
       for (int k = 0; k < D; ++k) {

@@ -146,6 +146,7 @@ struct StepArgs {
   uint32_t n_insts;
   uint32_t any_torque;
   uint32_t uniform_vol; // every grain has GF_UNIFORM_VOL
+  uint32_t ghost;       // index of the ghost record (partner of broken and padding slots)
 };
 
 // Adjoint of one time step (differentiable runs).  State of step t as the

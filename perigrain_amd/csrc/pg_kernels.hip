@@ -38,13 +38,14 @@ template <int D> struct Nbr {
 };
 
 // Load the partner data of a column word with 16-byte loads (3 in 3D, 2 in
-// 2D).  Broken and padding slots load the node's own record instead (always
-// a valid address), so the bond loop has no branches around its loads.
-// UVOL: the grain's nodes share one volume, passed in gvol.
+// 2D).  Broken and padding slots load the ghost record (a point far from
+// every node, so their discarded term stays finite), so the bond loop has no
+// branches around its loads.  UVOL: the grain's nodes share one volume,
+// passed in gvol.
 template <int D, bool UVOL>
 __device__ __forceinline__ void load_nbr(const double *R, const double *Vn, double gvol,
-                                         uint32_t cw, uint32_t self, Nbr<D> &nb) {
-  const uint32_t j = (cw & BCOL_BROKEN) ? self : cw;
+                                         uint32_t cw, uint32_t ghost, Nbr<D> &nb) {
+  const uint32_t j = (cw & BCOL_BROKEN) ? ghost : cw;
   const double2 *p = reinterpret_cast<const double2 *>(R + (size_t)j * (2 * D));
   if constexpr (D == 3) {
     const double2 a = p[0], b = p[1], c = p[2];
@@ -66,9 +67,10 @@ __device__ __forceinline__ void load_nbr(const double *R, const double *Vn, doub
 
 // One bond of get_peridynamic_force (particle2.h:125-149) and its break test
 // (remove_bonds, particle2.h:207), evaluated for every slot; broken and
-// padding slots add exactly +0.0 (the sum starts at +0.0 and can never
-// become -0.0, so adding +0.0 leaves it bit-identical).  A bond whose
-// stretch exceeds s0 gets the broken flag in its column word.
+// padding slots (ghost partner, finite values) get a zero weight and add a
+// zero of either sign (the sum starts at +0.0 and can never become -0.0, so
+// adding a zero leaves it bit-identical).  A bond whose stretch exceeds s0
+// gets the broken flag in its column word.
 template <int D>
 __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const double *Xo,
                                           const double *ui, double cnot, double snot, bool brk,
@@ -115,12 +117,9 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
 #pragma unroll
   for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
 #endif
-  const double w = (cnot * s) * nb.V;
+  const double w = live ? (cnot * s) * nb.V : 0.0;
 #pragma unroll
-  for (int q = 0; q < D; ++q) {
-    const double term = w * e[q];
-    F[q] = F[q] + (live ? term : 0.0);
-  }
+  for (int q = 0; q < D; ++q) F[q] = F[q] + w * e[q];
   if (live && brk && s > snot) {
     *slot = cw | BCOL_BROKEN;
     if (A.break_step) A.break_step[slot - A.bcol] = t;
@@ -234,18 +233,18 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 template <int D, bool UVOL> struct GlobalRecs {
   const double *U;
   const double *V;
-  __device__ __forceinline__ void load(uint32_t cw, uint32_t self, double gvol, Nbr<D> &nb) const {
-    load_nbr<D, UVOL>(U, V, gvol, cw, self, nb);
+  uint32_t ghost;
+  __device__ __forceinline__ void load(uint32_t cw, uint32_t, double gvol, Nbr<D> &nb) const {
+    load_nbr<D, UVOL>(U, V, gvol, cw, ghost, nb);
   }
 };
 template <int D, bool UVOL> struct LdsRecs {
-  const double *W; // LDS: records of nodes lo .. lo+cnt-1
-  uint32_t lo;
+  const double *W; // LDS: records of nodes lo .. lo+cnt-1, then the ghost
+  uint32_t lo, cnt, ghost;
   const double *V;
-  // Broken and padding slots read the window's first record: their term is
-  // computed and discarded (bond_term adds +0.0 for them).
+  // Broken and padding slots read the ghost record (after the window)
   __device__ __forceinline__ void load(uint32_t cw, uint32_t, double gvol, Nbr<D> &nb) const {
-    const uint32_t off = (cw & BCOL_BROKEN) ? 0u : cw - lo;
+    const uint32_t off = (cw & BCOL_BROKEN) ? cnt : cw - lo;
     const double2 *p = reinterpret_cast<const double2 *>(W + off * (2 * D));
     if constexpr (D == 3) {
       const double2 a = p[0], b = p[1], c = p[2];
@@ -262,7 +261,7 @@ template <int D, bool UVOL> struct LdsRecs {
       nb.u[0] = b.x;
       nb.u[1] = b.y;
     }
-    nb.V = UVOL ? gvol : V[(cw & BCOL_BROKEN) ? lo : cw];
+    nb.V = UVOL ? gvol : V[(cw & BCOL_BROKEN) ? ghost : cw];
   }
 };
 
@@ -519,7 +518,7 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
   const uint32_t lb = xcd_block(blockIdx.x, nb);
   const uint32_t i = lb * blockDim.x + threadIdx.x;
   step_housekeeping<D>(A, t, i, nb * blockDim.x);
-  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>{A.rec[t & 1], A.V});
+  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>{A.rec[t & 1], A.V, A.ghost});
 }
 
 // The same step with each workgroup's partner window staged in LDS: block b
@@ -540,11 +539,14 @@ __global__ __launch_bounds__(1024) void k_step_lds(const StepArgs A, const uint3
     double2 *dst = reinterpret_cast<double2 *>(lds_rec);
     const uint32_t n2 = cnt * D; // records are D double2 each
     for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x) dst[q] = src[q];
+    if (threadIdx.x < D) // the ghost record after the window
+      dst[n2 + threadIdx.x] =
+          reinterpret_cast<const double2 *>(A.rec[t & 1] + (size_t)A.ghost * 2 * D)[threadIdx.x];
   }
   __syncthreads();
   const uint32_t S = A.blk_slices;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  const LdsRecs<D, UVOL> rs{lds_rec, lo, A.V};
+  const LdsRecs<D, UVOL> rs{lds_rec, lo, cnt, A.ghost, A.V};
   for (uint32_t sl = wave; sl < S; sl += nw)
     step_node<D, RECORD, UVOL>(A, t, (lb * S + sl) * 64 + lane, rs);
 }
