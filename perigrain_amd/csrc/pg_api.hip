@@ -88,8 +88,10 @@ struct pg_handle {
   GrainDev *grains = nullptr;
   InstDev *insts = nullptr;
   uint32_t *d_slice_base = nullptr, *d_slice_width = nullptr, *d_bcol = nullptr;
-  uint32_t *cl_ptr = nullptr, *cl_j = nullptr;
+  uint32_t *cl_ptr = nullptr, *cl_j = nullptr, *cl_nfast = nullptr;
   int32_t *cl_slot = nullptr;
+  uint2 *cl_fast = nullptr;
+  uint8_t *brk_new = nullptr;
   size_t cl_cap = 0;
   unsigned long long *bbox[3] = {nullptr, nullptr, nullptr};
   double *wallpos[2] = {nullptr, nullptr};
@@ -215,6 +217,9 @@ void fill_args(pg_handle *h) {
   A.cl_ptr = h->cl_ptr;
   A.cl_j = h->cl_j;
   A.cl_slot = h->cl_slot;
+  A.cl_fast = h->cl_fast;
+  A.cl_nfast = h->cl_nfast;
+  A.brk_new = h->brk_new;
   A.xb = h->xb;
   for (int k = 0; k < 3; ++k) A.bbox[k] = h->bbox[k];
   A.wallpos[0] = h->wallpos[0];
@@ -270,6 +275,8 @@ int build_list(pg_handle *h, uint32_t ts, const double *rec_override = nullptr) 
   };
   if (!h->any_contact) {
     HIPCHK(hipMemsetAsync(h->cl_ptr, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemsetAsync(h->cl_nfast, 0, h->N * sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemsetAsync(h->brk_new, 0, h->N, h->stream));
     HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
     return PG_OK;
   }
@@ -337,20 +344,25 @@ int build_list(pg_handle *h, uint32_t ts, const double *rec_override = nullptr) 
   // 4. count, scan, fill the per-node lists
   HIPCHK(hipMemsetAsync(h->cl_cnt, 0, (h->N + 1) * sizeof(uint32_t), h->stream));
   HIPCHK(launch_list(D, false, args(), ts, h->grids, h->cell_start, h->cell_nodes, h->cl_cnt,
-                     nullptr, nullptr, nullptr, nullptr, h->stream));
+                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, h->stream));
   RC(scan(h, h->cl_cnt, h->cl_ptr, (size_t)h->N + 1));
   uint32_t total = 0;
   RC(d2h(h, &total, h->cl_ptr + h->N, 1));
   if (total > h->cl_cap) {
     RC(dfree(h, h->cl_j));
     RC(dfree(h, h->cl_slot));
+    RC(dfree(h, h->cl_fast));
     h->cl_cap = (size_t)(total * 1.5) + 1024;
     RC(dalloc(h, h->cl_j, h->cl_cap));
     RC(dalloc(h, h->cl_slot, h->cl_cap));
+    RC(dalloc(h, h->cl_fast, h->cl_cap));
     fill_args(h);
   }
   HIPCHK(launch_list(D, true, args(), ts, h->grids, h->cell_start, h->cell_nodes, nullptr,
-                     h->cl_ptr, h->cl_j, h->cl_slot, h->xb, h->stream));
+                     h->cl_ptr, h->cl_j, h->cl_slot, h->cl_fast, h->cl_nfast, h->xb,
+                     h->stream));
+  // the fast lists hold the bond states of the build: no bond broken since
+  HIPCHK(hipMemsetAsync(h->brk_new, 0, h->N, h->stream));
   HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
   h->list_entries = total;
   h->rebuilds += 1;
@@ -493,7 +505,6 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
       gd.rho = st.rho[g];
       gd.cnot = st.cnot[g];
       gd.snot = st.snot[g];
-      gd.yshift = 0.0;
       gd.flags = (st.movable[g] ? GF_MOVABLE : 0) | (st.breakable[g] ? GF_BREAKABLE : 0) |
                  (st.stoppable[g] ? GF_STOPPABLE : 0);
       gd.inst = s;
@@ -773,9 +784,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   }
   RC(dalloc(h, h->cl_ptr, (size_t)N + 1));
   RC(dalloc(h, h->cl_cnt, (size_t)N + 1));
+  RC(dalloc(h, h->cl_nfast, (size_t)N));
+  RC(dalloc(h, h->brk_new, (size_t)N));
   h->cl_cap = 1024;
   RC(dalloc(h, h->cl_j, h->cl_cap));
   RC(dalloc(h, h->cl_slot, h->cl_cap));
+  RC(dalloc(h, h->cl_fast, h->cl_cap));
   for (int k = 0; k < 3; ++k) RC(dalloc(h, h->bbox[k], (size_t)G * 2 * D));
   RC(dalloc(h, h->wallpos[0], (size_t)n * 6));
   RC(dalloc(h, h->wallpos[1], (size_t)n * 6));

@@ -143,13 +143,6 @@ __device__ __forceinline__ void rec_set_u(double *R, uint32_t i, const double *u
   for (int k = 0; k < D; ++k) r[k] = u[k];
 }
 
-// Current reference position (pos, including a reset_partzero_y shift).
-template <int D>
-__device__ __forceinline__ void cur_pos(const StepArgs &A, uint32_t i, const GrainDev &gp,
-                                        double *X) {
-  rec_X<D>(A, i, X);
-  if (gp.flags & GF_YSHIFT) X[1] = X[1] + gp.yshift;
-}
 template <int D>
 __device__ __forceinline__ void load_vec(const double *p, uint32_t ns, uint32_t i, double *v) {
 #pragma unroll

@@ -49,7 +49,6 @@ __host__ __device__ inline uint64_t bslot(uint64_t base, uint32_t lane, uint32_t
 constexpr int32_t GF_MOVABLE = 1;
 constexpr int32_t GF_BREAKABLE = 2;
 constexpr int32_t GF_STOPPABLE = 4;
-constexpr int32_t GF_YSHIFT = 8;       // pos(1) carries a reset_partzero_y shift
 constexpr int32_t GF_UNIFORM_VOL = 16; // every node of the grain has volume GrainDev::vol
 
 // instance flag bits
@@ -67,7 +66,7 @@ constexpr uint8_t NI_CLAMPED = 1;
 constexpr uint8_t NI_CONTACT = 2; // takes part in node-node and wall contact
 
 struct GrainDev {
-  double rho, cnot, snot, yshift;
+  double rho, cnot, snot;
   int32_t flags;
   int32_t inst;
   uint32_t node_begin, node_end;
@@ -123,7 +122,14 @@ struct StepArgs {
   uint32_t *bcol; // column words (partner index | BCOL_BROKEN), BCOL_PAD
   const uint32_t *cl_ptr;
   const uint32_t *cl_j;
-  const int32_t *cl_slot;
+  const int32_t *cl_slot;     // bond slot of a same-grain partner in the node's row, or -1
+  // Entries cl_ptr[i] .. cl_ptr[i] + cl_nfast[i] - 1 of cl_fast are the
+  // node's list entries without an intact bond at the list build, in list
+  // order, as (partner, partner grain).  A node reads them instead of its
+  // full list while none of its bonds has broken since the build (brk_new).
+  const uint2 *cl_fast;
+  const uint32_t *cl_nfast;
+  uint8_t *brk_new;           // [ns] a bond of the node broke since the last list build
   const double *xb;            // [dim][ns] positions at the last list build
   unsigned long long *bbox[3]; // ordered keys, [grains][2*dim] (min..., max...)
   double *wallpos[2];          // [insts][6]

@@ -28,7 +28,8 @@ hipError_t launch_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t 
                             uint32_t *cell_cur, uint32_t *cell_nodes, hipStream_t s);
 hipError_t launch_list(int dim, bool fill, const StepArgs &A, uint32_t t, const GridDev *grids,
                        const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
-                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb,
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, uint2 *lfast,
+                       uint32_t *nfast, double *xb,
                        hipStream_t s);
 hipError_t launch_pairs(int dim, bool fill, const StepArgs &A, uint32_t t, uint32_t *cnt,
                         const uint32_t *ptr, uint32_t *out, hipStream_t s);

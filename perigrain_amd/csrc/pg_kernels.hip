@@ -75,7 +75,7 @@ template <int D>
 __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const double *Xo,
                                           const double *ui, double cnot, double snot, bool brk,
                                           uint32_t *slot, double *F, const StepArgs &A,
-                                          uint32_t t) {
+                                          uint32_t t, bool &broke) {
   const bool live = !(cw & BCOL_BROKEN);
   double xi[D], eta[D], e[D];
 #pragma unroll
@@ -122,32 +122,27 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
   for (int q = 0; q < D; ++q) F[q] = F[q] + w * e[q];
   if (live && brk && s > snot) {
     *slot = cw | BCOL_BROKEN;
+    broke = true;
     if (A.break_step) A.break_step[slot - A.bcol] = t;
   }
 }
 
-// Contact partner test shared by the step kernel and the pair readback:
-// returns true and fills (d, r, R, K) if partner list entry c of node i is
-// in contact at step t (the gates of timeloop.h:698-747 and the self-contact
-// rules of contact.h:686-733).
+// Contact gates of a list entry (the gates of timeloop.h:698-747 and the
+// self-contact rules of contact.h:686-733): partner j of grain gj, reference
+// position Xj, bond slot bs (-1: no bond, or one known to be broken).
+// Returns whether the pair is evaluated at step t, with its contact radius R
+// and stiffness K.
 template <int D>
-__device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, uint32_t c,
-                                               uint32_t g, const GrainDev &gp,
-                                               const InstDev &ip, const double *xc,
-                                               const double *Xc, bool selfc, bool stop,
-                                               const double *boxg, const double *U,
-                                               uint32_t &j, double *d, double &r, double &R,
-                                               double &K) {
-  j = A.cl_j[c];
-  const uint32_t gj = A.node_grain[j];
+__device__ __forceinline__ bool contact_gate(const StepArgs &A, uint32_t t, uint32_t gj,
+                                            int32_t bs, uint32_t g, const GrainDev &gp,
+                                            const InstDev &ip, const double *Xc,
+                                            const double *Xj, bool selfc, bool stop,
+                                            const double *boxg, double &R, double &K) {
   R = ip.Rc;
   K = ip.Kn;
-  double Xj[D];
   if (gj == g) {
     if (!selfc) return false;
-    const int32_t bs = A.cl_slot[c];
     if (bs >= 0 && !(A.bcol[bs] & BCOL_BROKEN)) return false; // intact bond
-    cur_pos<D>(A, j, gp, Xj);
     double dX[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) dX[k] = Xc[k] - Xj[k];
@@ -159,36 +154,47 @@ __device__ __forceinline__ bool contact_partner(const StepArgs &A, uint32_t t, u
       R = ip.Rc;
       K = gp.cnot / ip.Rc;
     }
-  } else {
-    if (!stop) return false;
-    double boxj[2 * D];
-    load_box<D>(A.bbox[t % 3], gj, boxj);
-    if (!boxes_meet<D>(boxg, boxj, ip.Rc)) return false;
-    const GrainDev gpj = A.grains[gj];
-    cur_pos<D>(A, j, gpj, Xj);
+    return true;
   }
-  double uj[D];
-  rec_u<D>(U, j, uj);
+  if (!stop) return false;
+  double boxj[2 * D];
+  load_box<D>(A.bbox[t % 3], gj, boxj);
+  return boxes_meet<D>(boxg, boxj, ip.Rc);
+}
+
+// Relative position d = x_j - x_i and distance r of a pair; true if r <= R.
+template <int D>
+__device__ __forceinline__ bool pair_within(const double *Xj, const double *uj, const double *xc,
+                                           double R, double *d, double &r) {
 #pragma unroll
   for (int k = 0; k < D; ++k) d[k] = (Xj[k] + uj[k]) - xc[k];
   r = sqrt(red<D>(d, d));
   return r <= R;
 }
 
-// Wavefront min/max of D coordinates grouped by `grp` (uniform group fast
-// path), combined into keys[grp][2*D] with 64-bit atomics.
+// Wavefront min/max of D coordinates grouped by `grp`, combined into
+// keys[grp][2*D] with 64-bit atomics: one shuffle reduction and one atomic per
+// key for each distinct group in the wavefront (a wavefront that straddles
+// two grains makes two rounds), so that no key sees more than one atomic per
+// wavefront.  All 64 lanes call it together.
 template <int D>
 __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32_t grp,
                                                 bool valid, const double *x) {
-  unsigned long long kmin[D], kmax[D];
+  unsigned long long key[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    kmin[k] = valid ? dkey(x[k]) : ~0ull;
-    kmax[k] = valid ? dkey(x[k]) : 0ull;
-  }
-  const uint32_t g0 = __builtin_amdgcn_readfirstlane(grp);
-  const bool uniform = __all((!valid) || grp == g0);
-  if (uniform) {
+  for (int k = 0; k < D; ++k) key[k] = dkey(x[k]);
+  bool todo = valid && grp != 0xFFFFFFFFu;
+  for (;;) {
+    const unsigned long long m = __ballot(todo);
+    if (m == 0) break;
+    const uint32_t g0 = __builtin_amdgcn_readlane(grp, (int)__builtin_ctzll(m));
+    const bool in = todo && grp == g0;
+    unsigned long long kmin[D], kmax[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      kmin[k] = in ? key[k] : ~0ull;
+      kmax[k] = in ? key[k] : 0ull;
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
@@ -199,19 +205,14 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
         kmax[k] = b > kmax[k] ? b : kmax[k];
       }
     }
-    if ((threadIdx.x & 63) == 0 && g0 != 0xFFFFFFFFu) {
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         atomicMin(&keys[(size_t)g0 * 2 * D + k], kmin[k]);
         atomicMax(&keys[(size_t)g0 * 2 * D + D + k], kmax[k]);
       }
     }
-  } else if (valid) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      atomicMin(&keys[(size_t)grp * 2 * D + k], kmin[k]);
-      atomicMax(&keys[(size_t)grp * 2 * D + D + k], kmax[k]);
-    }
+    todo = todo && !in;
   }
 }
 
@@ -299,6 +300,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 #pragma unroll
     for (int k = 0; k < D; ++k) F[k] = 0.0;
 
+    bool broke = false; // a bond of node i broke in this step
     // ---- (a) PMB bond force with bond breaking.  Only what the bond loop
     // needs is live here; everything else is loaded after it.  The loop is
     // software-pipelined: the partner data of the next two bonds and the
@@ -339,16 +341,17 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
           uint32_t *sq = bc + (SLICE * BGROUP) * q;
           rs.load(cg.z, i, gvol, rb0);
           rs.load(cg.w, i, gvol, rb1);
-          bond_term<D>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F, A, t);
-          bond_term<D>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F, A, t);
+          bond_term<D>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F, A, t, broke);
+          bond_term<D>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F, A, t, broke);
           const uint4 cnn = colg(q + 2);
           rs.load(cn.x, i, gvol, ra0);
           rs.load(cn.y, i, gvol, ra1);
-          bond_term<D>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F, A, t);
-          bond_term<D>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F, A, t);
+          bond_term<D>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F, A, t, broke);
+          bond_term<D>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F, A, t, broke);
           cg = cn;
           cn = cnn;
         }
+        if (broke) A.brk_new[i] = 1;
       }
     }
 
@@ -357,7 +360,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], ui[D], xc[D];
     rec_u<D>(U, i, ui);
-    cur_pos<D>(A, i, gp, Xc);
+    rec_X<D>(A, i, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
     const uint8_t info = A.node_info[i];
@@ -384,17 +387,41 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
       double boxg[2 * D];
       load_box<D>(A.bbox[t % 3], g, boxg);
       if ((info & NI_CONTACT) && (selfc || stop)) {
-        const uint32_t c0 = A.cl_ptr[i], c1 = A.cl_ptr[i + 1];
+        // the fast list while no bond of the node has broken since the list
+        // build (its other entries are intact bonds, which stay excluded)
+        const uint32_t c0 = A.cl_ptr[i];
+        const bool full = broke || A.brk_new[i];
+        const uint32_t c1 = full ? A.cl_ptr[i + 1] : c0 + A.cl_nfast[i];
+        // the next fast-list entry is loaded while the current one is
+        // evaluated, and a partner's data is loaded before the gates decide
+        // whether it is needed (one round trip to memory per entry)
+        uint2 nxt = (!full && c0 < c1) ? A.cl_fast[c0] : make_uint2(0u, 0u);
         for (uint32_t c = c0; c < c1; ++c) {
-          uint32_t j;
+          uint32_t j, gj;
+          int32_t bs;
+          if (full) {
+            j = A.cl_j[c];
+            gj = A.node_grain[j];
+            bs = A.cl_slot[c];
+          } else {
+            j = nxt.x;
+            gj = nxt.y;
+            bs = -1;
+            if (c + 1 < c1) nxt = A.cl_fast[c + 1];
+          }
+          double Xj[D], uj[D], vj[D];
+          rec_X<D>(A, j, Xj);
+          rec_u<D>(U, j, uj);
+          load_vec<D>(vprev, ns, j, vj);
+          const double Vj = A.V[j];
           double d[D], r, R, K;
-          if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, boxg, U, j, d, r, R,
-                                  K))
+          if (!contact_gate<D>(A, t, gj, bs, g, gp, ip, Xc, Xj, selfc, stop, boxg, R, K) ||
+              !pair_within<D>(Xj, uj, xc, R, d, r))
             continue;
           double w[D];
 #pragma unroll
-          for (int k = 0; k < D; ++k) w[k] = vi[k] - vprev[(size_t)k * ns + j];
-          pair_force<D, double>(d, r, R, K, A.V[j], w, gp.rho, ip.mu, ip, F);
+          for (int k = 0; k < D; ++k) w[k] = vi[k] - vj[k];
+          pair_force<D, double>(d, r, R, K, Vj, w, gp.rho, ip.mu, ip, F);
         }
       }
 
@@ -567,7 +594,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], un[D], u0[D];
-    cur_pos<D>(A, i, gp, Xc);
+    rec_X<D>(A, i, Xc);
     rec_u<D>(A.rec[0], i, u0);
     const bool mov = grain_movable(gp, ip, g, 1);
 #pragma unroll
@@ -597,7 +624,7 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
   for (uint32_t e = gp.node_begin; e < gp.node_end; ++e) {
     const uint32_t n = A.ext2int[e];
     double Xc[D], un[D];
-    cur_pos<D>(A, n, gp, Xc);
+    rec_X<D>(A, n, Xc);
     rec_u<D>(U, n, un);
 #pragma unroll
     for (int k = 0; k < D; ++k) m[k] += (Xc[k] + un[k]);
@@ -677,7 +704,7 @@ __device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_
   if (!(A.node_info[i] & NI_CONTACT)) return false;
   const GrainDev gp = A.grains[A.node_grain[i]];
   double Xc[D], un[D];
-  cur_pos<D>(A, i, gp, Xc);
+  rec_X<D>(A, i, Xc);
   rec_u<D>(A.rec[t & 1], i, un);
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = Xc[k] + un[k];
@@ -746,16 +773,20 @@ __global__ void k_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t 
 // Count (FILL=false) or write (FILL=true) the Verlet-list entries of node i:
 // every other contact node of the same instance within the list radius at
 // build time.
+// The fill pass also writes the node's fast list: its entries without an
+// intact bond, as (partner, partner grain), and their number.
 template <int D, bool FILL>
 __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
                        const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
-                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb) {
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, uint2 *lfast,
+                       uint32_t *nfast, double *xb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n_nodes) return;
   double x[D];
   uint32_t inst;
   if (!contact_x<D>(A, t, i, x, inst)) {
     if (!FILL) cnt[i] = 0;
+    else nfast[i] = 0;
     return;
   }
   const GridDev gd = grids[inst];
@@ -816,22 +847,28 @@ __global__ void k_list(const StepArgs A, const uint32_t t, const GridDev *grids,
   // bond slot of same-grain partners, for the intact-bond exclusion
   const uint32_t sl = i >> 6, lane = i & 63;
   const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+  uint32_t nf = 0;
   for (uint32_t a = 0; a < n; ++a) {
     const uint32_t j = lj[out0 + a];
+    const uint32_t gj = A.node_grain[j];
     int32_t slot = -1;
-    if (A.node_grain[j] == g) {
+    bool intact = false;
+    if (gj == g) {
       for (uint32_t k = 0; k < width; ++k) {
         const uint32_t e = (uint32_t)bslot(base, lane, k);
         const uint32_t w = A.bcol[e];
         if (w == BCOL_PAD) break;
         if ((w & BCOL_INDEX) == j) {
           slot = (int32_t)e;
+          intact = !(w & BCOL_BROKEN);
           break;
         }
       }
     }
     lslot[out0 + a] = slot;
+    if (!intact) lfast[out0 + nf++] = make_uint2(j, gj);
   }
+  nfast[i] = nf;
 }
 
 // Contact pairs of the completed step t (readback for parity checks).
@@ -852,16 +889,20 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
   if (mov && (info & NI_CONTACT) && (selfc || stop)) {
     const double *U = A.rec[t & 1];
     double Xc[D], xc[D], box[2 * D], ui[D];
-    cur_pos<D>(A, i, gp, Xc);
+    rec_X<D>(A, i, Xc);
     rec_u<D>(U, i, ui);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k];
     load_box<D>(A.bbox[t % 3], g, box);
     const uint32_t o = FILL ? ptr[i] : 0;
     for (uint32_t c = A.cl_ptr[i]; c < A.cl_ptr[i + 1]; ++c) {
-      uint32_t j;
-      double d[D], r, R, K;
-      if (!contact_partner<D>(A, t, c, g, gp, ip, xc, Xc, selfc, stop, box, U, j, d, r, R, K))
+      const uint32_t j = A.cl_j[c];
+      double Xj[D], uj[D], d[D], r, R, K;
+      rec_X<D>(A, j, Xj);
+      rec_u<D>(U, j, uj);
+      if (!contact_gate<D>(A, t, A.node_grain[j], A.cl_slot[c], g, gp, ip, Xc, Xj, selfc, stop,
+                           box, R, K) ||
+          !pair_within<D>(Xj, uj, xc, R, d, r))
         continue;
       if (FILL) {
         out[2 * (o + n)] = i;
@@ -1004,24 +1045,25 @@ hipError_t launch_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t 
 
 hipError_t launch_list(int dim, bool fill, const StepArgs &A, uint32_t t, const GridDev *grids,
                        const uint32_t *cell_start, const uint32_t *cell_nodes, uint32_t *cnt,
-                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, double *xb,
+                       const uint32_t *ptr, uint32_t *lj, int32_t *lslot, uint2 *lfast,
+                       uint32_t *nfast, double *xb,
                        hipStream_t s) {
   const uint32_t nb = nblk(A.n_nodes, 128);
   if (nb == 0) return hipSuccess;
   if (dim == 2) {
     if (fill)
       hipLaunchKernelGGL((k_list<2, true>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
-                         cell_nodes, cnt, ptr, lj, lslot, xb);
+                         cell_nodes, cnt, ptr, lj, lslot, lfast, nfast, xb);
     else
       hipLaunchKernelGGL((k_list<2, false>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
-                         cell_nodes, cnt, ptr, lj, lslot, xb);
+                         cell_nodes, cnt, ptr, lj, lslot, lfast, nfast, xb);
   } else {
     if (fill)
       hipLaunchKernelGGL((k_list<3, true>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
-                         cell_nodes, cnt, ptr, lj, lslot, xb);
+                         cell_nodes, cnt, ptr, lj, lslot, lfast, nfast, xb);
     else
       hipLaunchKernelGGL((k_list<3, false>), dim3(nb), dim3(128), 0, s, A, t, grids, cell_start,
-                         cell_nodes, cnt, ptr, lj, lslot, xb);
+                         cell_nodes, cnt, ptr, lj, lslot, lfast, nfast, xb);
   }
   return hipGetLastError();
 }
