@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
       force_cotangent<D>(ip, gp.rho, Pu, Pv, Pa, lv, lF);
       double Xi[D], ui[D], xi[D], vi[D], au[D], av[D];
       rec_X<D>(A, i, Xi);
-      rec_u<D>(B.su, i, ui);
+      rec_u<D>(B.su, A.ns, i, ui);
       load_cols<D>(B.sv, ns, i, vi);
 #pragma unroll
       for (int k = 0; k < D; ++k) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
           const uint32_t j = cw & BCOL_INDEX;
           double Xj[D], uj[D], lFj[D], xb[D], eta[D], e[D], q[D];
           rec_X<D>(A, j, Xj);
-          rec_u<D>(B.su, j, uj);
+          rec_u<D>(B.su, A.ns, j, uj);
           load_cols<D>(B.lf_in, ns, j, lFj);
 #pragma unroll
           for (int r = 0; r < D; ++r) {
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
           const int32_t bs = A.cl_slot[cc];
           double Xj[D], uj[D], xj[D], vj[D];
           rec_X<D>(A, j, Xj);
-          rec_u<D>(B.su, j, uj);
+          rec_u<D>(B.su, A.ns, j, uj);
           load_cols<D>(B.sv, ns, j, vj);
 #pragma unroll
           for (int k = 0; k < D; ++k) xj[k] = Xj[k] + uj[k];
@@ -320,11 +320,11 @@ __global__ void k_adj_maxdisp(const StepArgs A, const double *su, uint32_t steps
   if (i >= A.n_nodes || !(A.node_info[i] & NI_CONTACT)) return;
   const size_t stride = (size_t)A.ns * 2 * D;
   double u0[D];
-  rec_u<D>(su, i, u0);
+  rec_u<D>(su, A.ns, i, u0);
   double m = 0.0;
   for (uint32_t s = 1; s < steps; ++s) {
     double u[D], dd[D];
-    rec_u<D>(su + s * stride, i, u);
+    rec_u<D>(su + s * stride, A.ns, i, u);
 #pragma unroll
     for (int k = 0; k < D; ++k) dd[k] = u[k] - u0[k];
     const double r = sqrt(red<D>(dd, dd));

@@ -453,7 +453,8 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
     ih.nsides = D == 2 ? 4 : 6;
   }
-  if (N >= 0x7FFFFF00ull) return fail(PG_ERR_INVALID, "too many nodes in batch");
+  // record gathers use 32-bit byte offsets of 16 bytes per node
+  if (N >= (1ull << 28) - 64) return fail(PG_ERR_INVALID, "too many nodes in batch (limit 2^28 - 64)");
   h->N = (uint32_t)N;
   h->G = (uint32_t)G;
   h->I = (uint32_t)n;
@@ -809,19 +810,19 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     h->h_pin = static_cast<uint32_t *>(p);
   }
 
-  // ---- uploads (records node-major, the rest transposed to component-major)
+  // ---- uploads (records in parts, the rest transposed to component-major)
   {
     const uint32_t nsd = h->ns;
     std::vector<double> sR(2 * ND, 0.0), sV(nsd, 0.0), sVel(ND, 0.0), sAcc(ND, 0.0),
         sExt(ND, 0.0);
     // the ghost: far from every node (1e30 in each coordinate), volume 1
-    for (int k = 0; k < D; ++k) sR[N * 2 * D + k] = 1e30;
+    for (int k = 0; k < D; ++k) sR[rec_index(nsd, N, k)] = 1e30;
     sV[N] = 1.0;
     for (uint64_t i = 0; i < N; ++i) {
       sV[i] = rec[i].X[D];
       for (int k = 0; k < D; ++k) {
-        sR[i * 2 * D + k] = rec[i].X[k];
-        sR[i * 2 * D + D + k] = rec[i].u[k];
+        sR[rec_index(nsd, i, k)] = rec[i].X[k];
+        sR[rec_index(nsd, i, D + k)] = rec[i].u[k];
         sVel[(size_t)k * nsd + i] = vel[i * D + k];
         sAcc[(size_t)k * nsd + i] = acc[i * D + k];
         sExt[(size_t)k * nsd + i] = ext[i * D + k];
@@ -1181,13 +1182,16 @@ int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
     return d2h(h, v.data(), base + (size_t)k * h->ns + ih.node_begin, n);
   };
   if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
-    // records [node][2*D]: X then u; CurrPos = pos + disp
-    std::vector<double> r(n * 2 * D);
-    RC(d2h(h, r.data(), h->rec[h->t & 1] + (size_t)ih.node_begin * 2 * D, r.size()));
+    // record parts [D][ns][2] of the instance's node range; CurrPos = pos + disp
+    std::vector<double> r((size_t)D * n * 2);
+    for (int q = 0; q < D; ++q)
+      RC(d2h(h, r.data() + (size_t)q * n * 2,
+             h->rec[h->t & 1] + rec_index(h->ns, ih.node_begin, 2 * q), n * 2));
+    auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
     for (size_t e = 0; e < n; ++e) {
-      const double *q = &r[at(e) * 2 * D];
+      const size_t a = at(e);
       for (int k = 0; k < D; ++k)
-        dst[e * D + k] = field == PG_FIELD_CURRPOS ? q[k] + q[D + k] : q[D + k];
+        dst[e * D + k] = field == PG_FIELD_CURRPOS ? el(a, k) + el(a, D + k) : el(a, D + k);
     }
     return PG_OK;
   }

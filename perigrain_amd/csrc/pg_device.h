@@ -119,28 +119,42 @@ __device__ __forceinline__ bool grain_stoppable(const GrainDev &gp, const InstDe
 
 // Step index of a launch: the argument itself, or, for launches replayed
 // from a captured graph, the argument added to the device-side base step.
+// The value is the same in every lane; readfirstlane lets the compiler keep
+// it, and what derives from it (record array bases), in scalar registers.
 __device__ __forceinline__ uint32_t step_index(const StepArgs &A, uint32_t t_arg) {
-  return A.tbase ? *A.tbase + t_arg : t_arg;
+  return __builtin_amdgcn_readfirstlane(A.tbase ? *A.tbase + t_arg : t_arg);
 }
 
-// Record accessors: records are [node][2*D], X then u.
+// Global-memory byte pointer that is the same in every lane, kept in scalar
+// registers: loads from it plus a 32-bit lane offset use the scalar-base
+// addressing of global loads (no 64-bit address arithmetic per lane).
+typedef const char __attribute__((address_space(1))) *gbytes;
+typedef double __attribute__((ext_vector_type(2))) dvec2;
+__device__ __forceinline__ gbytes uniform_global(const void *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<gbytes>(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ dvec2 load_dvec2(gbytes base, uint32_t off) {
+  return *reinterpret_cast<const dvec2 __attribute__((address_space(1))) *>(base + off);
+}
+
+// Record accessors (records are [D][ns][2], rec_index).
 template <int D>
 __device__ __forceinline__ void rec_X(const StepArgs &A, uint32_t i, double *X) {
-  const double *r = A.rec[0] + (size_t)i * (2 * D); // X is stored in both parities
 #pragma unroll
-  for (int k = 0; k < D; ++k) X[k] = r[k];
+  for (int k = 0; k < D; ++k) X[k] = A.rec[0][rec_index(A.ns, i, k)]; // X: both parities
 }
 template <int D>
-__device__ __forceinline__ void rec_u(const double *R, uint32_t i, double *u) {
-  const double *r = R + (size_t)i * (2 * D) + D;
+__device__ __forceinline__ void rec_u(const double *R, uint32_t ns, uint32_t i, double *u) {
 #pragma unroll
-  for (int k = 0; k < D; ++k) u[k] = r[k];
+  for (int k = 0; k < D; ++k) u[k] = R[rec_index(ns, i, D + k)];
 }
 template <int D>
-__device__ __forceinline__ void rec_set_u(double *R, uint32_t i, const double *u) {
-  double *r = R + (size_t)i * (2 * D) + D;
+__device__ __forceinline__ void rec_set_u(double *R, uint32_t ns, uint32_t i, const double *u) {
 #pragma unroll
-  for (int k = 0; k < D; ++k) r[k] = u[k];
+  for (int k = 0; k < D; ++k) R[rec_index(ns, i, D + k)] = u[k];
 }
 
 template <int D>

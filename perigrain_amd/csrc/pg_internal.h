@@ -2,17 +2,20 @@
 // gfx950 kernels.
 //
 // Layout in HBM (one handle may pack several independent instances):
-//  * nodes are packed instance by instance and grain by grain; inside a
-//    grain they are stored in degree order within small windows (see
-//    pg_api.hip), and perm/inv map between that device order and the
-//    reference's node order;
+//  * nodes are packed instance by instance and grain by grain, in the
+//    reference's node order unless a degree-sort window is set (see
+//    pg_api.hip); perm/inv map between the device order and the reference's;
 //  * what a bond gathers from its partner -- reference position X and
-//    displacement u -- sits in one record of 2*dim doubles (48 B in 3D, 32 B
-//    in 2D), loaded with 16-byte loads (3 or 2 per bond); the partner volume
-//    comes from the grain when all its nodes share one volume (lattice
-//    grains), else from V.  Records are double-buffered by step parity
-//    (X is stored in both): partners read u of step t while a node writes
-//    u of step t+1;
+//    displacement u, 2*dim doubles, X then u -- is a record split into dim
+//    parts of two doubles: part q of node i is elements 2q, 2q+1 at
+//    rec[(q*ns + i)*2] (3D: (X0,X1) (X2,u0) (u1,u2); 2D: (X0,X1) (u0,u1)).
+//    A gather is dim 16-byte loads at the same byte offset 16*i from the
+//    part bases; each load of a wavefront covers the 16-byte slots of 64
+//    partners, which share cache lines far more often than 48-byte
+//    records do.  The partner volume comes from the grain when all its
+//    nodes share one volume (lattice grains), else from V.  Records are
+//    double-buffered by step parity (X is stored in both): partners read u
+//    of step t while a node writes u of step t+1;
 //  * the other node data are structure-of-arrays, component-major with a
 //    padded stride `ns` (vel[k*ns + i]), vel double-buffered like u;
 //  * bonds are sliced ELL with slices of 64 rows (one wavefront) and rows
@@ -103,10 +106,16 @@ struct GridDev {
   double r2; // squared Verlet-list radius (contact radius + skin, with margin)
 };
 
+// Element m (0..2*dim-1: X then u) of node i's record in a record array.
+__host__ __device__ inline size_t rec_index(uint32_t ns, size_t i, int m) {
+  return ((size_t)(m >> 1) * ns + i) * 2 + (m & 1);
+}
+
 // Everything the kernels touch (passed by value as kernarg).  Records are
-// [node][2*dim] (X then u); the other node arrays are [dim][ns].
+// [dim][ns][2] (parts of X then u, rec_index); the other node arrays are
+// [dim][ns].
 struct StepArgs {
-  double *rec[2];     // [n][2*dim] pos and disp, by step parity
+  double *rec[2];     // [dim][ns][2] pos and disp, by step parity
   const double *V;    // [ns] nodal volumes
   double *vel[2];     // [dim][ns] velocity, by step parity
   double *acc;        // [dim][ns]
@@ -160,7 +169,7 @@ struct StepArgs {
 // partial sums.  Node vectors are [dim][ns] unless noted.
 struct AdjArgs {
   StepArgs A;                  // static data; A.cl_* is the segment's candidate list
-  const double *su;            // [n][2*dim] records holding u^t
+  const double *su;            // records holding u^t
   const double *sv;            // v^{t-1}
   const double *swall;         // [insts][6] wall positions of step t
   const unsigned long long *sbox; // [grains][2*dim] grain boxes of step t

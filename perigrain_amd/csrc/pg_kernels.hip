@@ -37,18 +37,22 @@ template <int D> struct Nbr {
   double u[D];
 };
 
-// Load the partner data of a column word with 16-byte loads (3 in 3D, 2 in
-// 2D).  Broken and padding slots load the ghost record (a point far from
-// every node, so their discarded term stays finite), so the bond loop has no
-// branches around its loads.  UVOL: the grain's nodes share one volume,
-// passed in gvol.
+// Load the partner data of a column word: one 16-byte load from each record
+// part (3 in 3D, 2 in 2D), all at the byte offset 16*j from the part bases
+// (node indices stay below 2^28, so the offset fits 32 bits and the part
+// bases stay scalar).  Broken and padding slots load the ghost record (a
+// point far from every node, so their discarded term stays finite), so the
+// bond loop has no branches around its loads.  UVOL: the grain's nodes
+// share one volume, passed in gvol.
 template <int D, bool UVOL>
-__device__ __forceinline__ void load_nbr(const double *R, const double *Vn, double gvol,
+__device__ __forceinline__ void load_nbr(const gbytes *P, const double *Vn, double gvol,
                                          uint32_t cw, uint32_t ghost, Nbr<D> &nb) {
   const uint32_t j = (cw & BCOL_BROKEN) ? ghost : cw;
-  const double2 *p = reinterpret_cast<const double2 *>(R + (size_t)j * (2 * D));
+  const uint32_t off = j * 16u;
+  const dvec2 a = load_dvec2(P[0], off);
+  const dvec2 b = load_dvec2(P[1], off);
   if constexpr (D == 3) {
-    const double2 a = p[0], b = p[1], c = p[2];
+    const dvec2 c = load_dvec2(P[2], off);
     nb.X[0] = a.x;
     nb.X[1] = a.y;
     nb.X[2] = b.x;
@@ -56,7 +60,6 @@ __device__ __forceinline__ void load_nbr(const double *R, const double *Vn, doub
     nb.u[1] = c.x;
     nb.u[2] = c.y;
   } else {
-    const double2 a = p[0], b = p[1];
     nb.X[0] = a.x;
     nb.X[1] = a.y;
     nb.u[0] = b.x;
@@ -232,11 +235,17 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
 // Where the bond loop reads partner records from: the global record array
 // of step t, or a workgroup's window of it staged in LDS.
 template <int D, bool UVOL> struct GlobalRecs {
-  const double *U;
+  gbytes P[D]; // the record parts of step t
   const double *V;
   uint32_t ghost;
+  __device__ __forceinline__ GlobalRecs(const double *U, uint32_t ns, const double *V_,
+                                        uint32_t ghost_)
+      : V(V_), ghost(ghost_) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) P[q] = uniform_global(U + (size_t)q * ns * 2);
+  }
   __device__ __forceinline__ void load(uint32_t cw, uint32_t, double gvol, Nbr<D> &nb) const {
-    load_nbr<D, UVOL>(U, V, gvol, cw, ghost, nb);
+    load_nbr<D, UVOL>(P, V, gvol, cw, ghost, nb);
   }
 };
 template <int D, bool UVOL> struct LdsRecs {
@@ -315,7 +324,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
       if (mov_a) {
         double Xo[D], ui[D];
         rec_X<D>(A, i, Xo);
-        rec_u<D>(U, i, ui);
+        rec_u<D>(U, A.ns, i, ui);
         const double cnot = gq->cnot, snot = gq->snot;
         const double gvol = UVOL ? gq->vol : 0.0;
         const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
@@ -359,7 +368,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], ui[D], xc[D];
-    rec_u<D>(U, i, ui);
+    rec_u<D>(U, A.ns, i, ui);
     rec_X<D>(A, i, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
@@ -411,7 +420,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
           }
           double Xj[D], uj[D], vj[D];
           rec_X<D>(A, j, Xj);
-          rec_u<D>(U, j, uj);
+          rec_u<D>(U, A.ns, j, uj);
           load_vec<D>(vprev, ns, j, vj);
           const double Vj = A.V[j];
           double d[D], r, R, K;
@@ -510,7 +519,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 #pragma unroll
       for (int k = 0; k < D; ++k) un[k] = ui[k];
     }
-    rec_set_u<D>(A.rec[(t + 1) & 1], i, un);
+    rec_set_u<D>(A.rec[(t + 1) & 1], A.ns, i, un);
 #pragma unroll
     for (int k = 0; k < D; ++k) xn[k] = Xc[k] + un[k];
     if (info & NI_CONTACT) {
@@ -545,7 +554,7 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
   const uint32_t lb = xcd_block(blockIdx.x, nb);
   const uint32_t i = lb * blockDim.x + threadIdx.x;
   step_housekeeping<D>(A, t, i, nb * blockDim.x);
-  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>{A.rec[t & 1], A.V, A.ghost});
+  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>(A.rec[t & 1], A.ns, A.V, A.ghost));
 }
 
 // The same step with each workgroup's partner window staged in LDS: block b
@@ -562,13 +571,17 @@ __global__ __launch_bounds__(1024) void k_step_lds(const StepArgs A, const uint3
   step_housekeeping<D>(A, t, lb * blockDim.x + threadIdx.x, nb * blockDim.x);
   const uint32_t lo = A.win_lo[lb], cnt = A.win_cnt[lb];
   {
-    const double2 *src = reinterpret_cast<const double2 *>(A.rec[t & 1] + (size_t)lo * 2 * D);
+    // LDS holds whole records [cnt + 1][D] double2 (the ghost last), copied
+    // part by part from the record parts of nodes lo .. lo + cnt - 1
+    const double2 *src = reinterpret_cast<const double2 *>(A.rec[t & 1]);
     double2 *dst = reinterpret_cast<double2 *>(lds_rec);
-    const uint32_t n2 = cnt * D; // records are D double2 each
-    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x) dst[q] = src[q];
+    const uint32_t n2 = cnt * D;
+    for (uint32_t q = threadIdx.x; q < n2; q += blockDim.x) {
+      const uint32_t part = q / cnt, r = q - part * cnt;
+      dst[r * D + part] = src[(size_t)part * A.ns + lo + r];
+    }
     if (threadIdx.x < D) // the ghost record after the window
-      dst[n2 + threadIdx.x] =
-          reinterpret_cast<const double2 *>(A.rec[t & 1] + (size_t)A.ghost * 2 * D)[threadIdx.x];
+      dst[n2 + threadIdx.x] = src[(size_t)threadIdx.x * A.ns + A.ghost];
   }
   __syncthreads();
   const uint32_t S = A.blk_slices;
@@ -595,7 +608,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], un[D], u0[D];
     rec_X<D>(A, i, Xc);
-    rec_u<D>(A.rec[0], i, u0);
+    rec_u<D>(A.rec[0], A.ns, i, u0);
     const bool mov = grain_movable(gp, ip, g, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -604,7 +617,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
       un[k] = mov ? u0[k] + ((ip.dt * v) + (ip.h * a)) : u0[k];
       xn[k] = Xc[k] + un[k];
     }
-    rec_set_u<D>(A.rec[1], i, un);
+    rec_set_u<D>(A.rec[1], A.ns, i, un);
   }
   wave_box_atomic<D>(A.bbox[1], g, valid, xn);
 }
@@ -625,7 +638,7 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
     const uint32_t n = A.ext2int[e];
     double Xc[D], un[D];
     rec_X<D>(A, n, Xc);
-    rec_u<D>(U, n, un);
+    rec_u<D>(U, A.ns, n, un);
 #pragma unroll
     for (int k = 0; k < D; ++k) m[k] += (Xc[k] + un[k]);
   }
@@ -705,7 +718,7 @@ __device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_
   const GrainDev gp = A.grains[A.node_grain[i]];
   double Xc[D], un[D];
   rec_X<D>(A, i, Xc);
-  rec_u<D>(A.rec[t & 1], i, un);
+  rec_u<D>(A.rec[t & 1], A.ns, i, un);
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = Xc[k] + un[k];
   inst = gp.inst;
@@ -890,7 +903,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
     const double *U = A.rec[t & 1];
     double Xc[D], xc[D], box[2 * D], ui[D];
     rec_X<D>(A, i, Xc);
-    rec_u<D>(U, i, ui);
+    rec_u<D>(U, A.ns, i, ui);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k];
     load_box<D>(A.bbox[t % 3], g, box);
@@ -899,7 +912,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
       const uint32_t j = A.cl_j[c];
       double Xj[D], uj[D], d[D], r, R, K;
       rec_X<D>(A, j, Xj);
-      rec_u<D>(U, j, uj);
+      rec_u<D>(U, A.ns, j, uj);
       if (!contact_gate<D>(A, t, A.node_grain[j], A.cl_slot[c], g, gp, ip, Xc, Xj, selfc, stop,
                            box, R, K) ||
           !pair_within<D>(Xj, uj, xc, R, d, r))
