@@ -94,6 +94,11 @@ typedef struct pg_params {
   int32_t set_movable_timestep;
   int32_t set_stoppable_index;   /* -1 = unused */
   int32_t set_stoppable_timestep;
+  /* timeloop.h:629-647: at step reset_partzero_y_timestep, move the first
+   * grain's pos in y so that its mean CurrPos y is the other grains' top y
+   * plus wheel_rad plus the contact radius (bonds keep their reference
+   * vectors).  Needs a second grain with at least two nodes; not with the
+   * adjoint entry points. */
   int32_t reset_partzero_y;
   uint32_t reset_partzero_y_timestep;
   double wheel_rad;

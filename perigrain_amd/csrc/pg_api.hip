@@ -55,6 +55,11 @@ struct InstHost {
   double Rc, skin;
   bool contact;
   int nsides;
+  // reset_partzero_y: the step at which the first grain is moved (0: never),
+  // the config's wheel_rad, and whether it happened
+  uint32_t reset_step = 0;
+  double wheel_rad = 0.0;
+  bool reset_done = false;
 };
 
 } // namespace
@@ -411,8 +416,10 @@ int check_setup(const pg_setup *s, const pg_params *p, int dim_expected) {
   if (s->node_offset[s->n_grains] >= 0x7FFFFF00ll)
     return fail(PG_ERR_INVALID, "too many nodes");
   if (!(p->dt > 0)) return fail(PG_ERR_INVALID, "dt must be > 0");
-  if (p->reset_partzero_y)
-    return fail(PG_ERR_INVALID, "reset_partzero_y is not supported by the HIP stepper");
+  if (p->reset_partzero_y && p->reset_partzero_y_timestep > 0 &&
+      (s->n_grains < 2 || s->node_offset[2] - s->node_offset[1] < 2))
+    return fail(PG_ERR_INVALID,
+                "reset_partzero_y needs a second grain with at least two nodes (timeloop.h:632)");
   auto chk_idx = [&](int32_t idx, int32_t ts) -> bool {
     return idx == -1 || ts <= 0 || (idx >= 0 && idx < s->n_grains);
   };
@@ -452,6 +459,9 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     ih.contact = st.contact_radius > 0;
     ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
     ih.nsides = D == 2 ? 4 : 6;
+    const pg_params &pp = params[s];
+    ih.reset_step = pp.reset_partzero_y ? pp.reset_partzero_y_timestep : 0;
+    ih.wheel_rad = pp.wheel_rad;
   }
   // record gathers use 32-bit byte offsets of 16 bytes per node
   if (N >= (1ull << 28) - 64) return fail(PG_ERR_INVALID, "too many nodes in batch (limit 2^28 - 64)");
@@ -574,6 +584,8 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     id.ckey_begin = ih.ckey_begin;
     id.ckey_end = ih.ckey_end;
     id.nsides = ih.nsides;
+    id.yshift_grain = -1;
+    id.yshift = 0.0;
     const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
     id.skin_half2 = sh * sh;
     if (pp.enable_torque) h->any_torque = true;
@@ -916,12 +928,72 @@ int ensure_graph(pg_handle *h) {
   return PG_OK;
 }
 
+size_t rec_len(const pg_handle *h) { return (size_t)h->ns * 2 * h->dim; }
+size_t vec_len(const pg_handle *h) { return (size_t)h->ns * h->dim; }
+size_t box_len(const pg_handle *h) { return (size_t)h->G * 2 * h->dim; }
+
+// reset_partzero_y of instance s before step tr (timeloop.h:629-647): the
+// largest y among the other grains' nodes of pos + disp of step tr-1
+// (starting, as the reference does, from pos of node 0 plus disp of node 1 of
+// the second grain) plus wheel_rad plus the contact radius is where the
+// first grain's mean CurrPos y goes; the difference is added to that grain's
+// pos y.  Computed on the host from the device state, once per run.
+int do_reset(pg_handle *h, uint32_t s, uint32_t tr) {
+  const int D = h->dim;
+  InstHost &ih = h->inst[s];
+  const size_t n = ih.node_end - ih.node_begin;
+  // record parts holding element 1 (X y) and element D+1 (u y, of step tr-1)
+  const int mu = D + 1;
+  std::vector<double> px(2 * n), pu(2 * n);
+  RC(d2h(h, px.data(), h->rec[0] + rec_index(h->ns, ih.node_begin, 0), 2 * n));
+  RC(d2h(h, pu.data(), h->rec[(tr - 1) & 1] + rec_index(h->ns, ih.node_begin, mu & ~1), 2 * n));
+  auto dev = [&](uint32_t e) -> size_t { return h->inv[e] - ih.node_begin; };
+  auto y = [&](size_t a_pos, size_t a_disp) { return px[2 * a_pos + 1] + pu[2 * a_disp + (mu & 1)]; };
+  const uint32_t g0 = ih.grain_begin, g1 = ih.grain_begin + 1;
+  double max_y = y(dev(h->grain_node_begin[g1]), dev(h->grain_node_begin[g1] + 1));
+  for (uint32_t e = h->grain_node_begin[g1]; e < h->grain_node_begin[ih.grain_end]; ++e) {
+    const double v = y(dev(e), dev(e));
+    if (v > max_y) max_y = v;
+  }
+  double mean = 0.0;
+  for (uint32_t e = h->grain_node_begin[g0]; e < h->grain_node_begin[g0 + 1]; ++e)
+    mean += y(dev(e), dev(e));
+  mean = mean / (double)(h->grain_node_begin[g0 + 1] - h->grain_node_begin[g0]);
+  const double dest = max_y + ih.wheel_rad + ih.Rc;
+  InstDev &id = h->inst_dev[s];
+  id.yshift_grain = (int32_t)g0;
+  id.yshift = dest - mean;
+  RC(h2d(h, h->insts + s, &id, 1));
+  ih.reset_done = true;
+  return PG_OK;
+}
+
+// First step after t at which some instance's reset_partzero_y is due, or 0.
+uint32_t next_reset(const pg_handle *h) {
+  uint32_t r = 0;
+  for (const InstHost &ih : h->inst)
+    if (ih.reset_step > h->t && !ih.reset_done && (r == 0 || ih.reset_step < r)) r = ih.reset_step;
+  return r;
+}
+
 int run_steps(pg_handle *h, uint32_t nsteps) {
   if (nsteps == 0) return PG_OK;
   const uint32_t t_end = h->t + nsteps;
   if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
   while (h->t < t_end) {
-    const uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
+    uint32_t tr = next_reset(h);
+    if (tr == h->t + 1) {
+      // the moved grain's positions change: boxes of step tr and a new list
+      for (uint32_t s = 0; s < h->I; ++s)
+        if (h->inst[s].reset_step == tr && !h->inst[s].reset_done) RC(do_reset(h, s, tr));
+      HIPCHK(launch_fill_box(h->bbox[tr % 3], box_len(h), h->dim, h->stream));
+      HIPCHK(launch_boxes(h->dim, h->A, tr, h->bbox[tr % 3], h->stream));
+      RC(build_list(h, tr));
+      tr = next_reset(h); // the next one due, after tr
+    }
+    // a chunk ends before the next reset
+    uint32_t chunk = std::min<uint32_t>(t_end - h->t, h->check_every);
+    if (tr > h->t + 1) chunk = std::min<uint32_t>(chunk, tr - 1 - h->t);
     // a whole chunk without the recording step can be replayed from a graph
     const bool replay =
         h->use_graph && !h->hist && chunk == h->check_every && h->t + chunk < t_end;
@@ -969,9 +1041,6 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
 
 // ------------------------------------------------------------ adjoint
 
-size_t rec_len(const pg_handle *h) { return (size_t)h->ns * 2 * h->dim; }
-size_t vec_len(const pg_handle *h) { return (size_t)h->ns * h->dim; }
-size_t box_len(const pg_handle *h) { return (size_t)h->G * 2 * h->dim; }
 
 template <class T> int d2d(pg_handle *h, T *dst, const T *src, size_t n) {
   if (n == 0) return PG_OK;
@@ -1188,10 +1257,20 @@ int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
       RC(d2h(h, r.data() + (size_t)q * n * 2,
              h->rec[h->t & 1] + rec_index(h->ns, ih.node_begin, 2 * q), n * 2));
     auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
+    // pos of the grain moved by reset_partzero_y carries the y shift
+    const InstDev &id = h->inst_dev[inst];
+    size_t s0 = 0, s1 = 0;
+    if (id.yshift_grain >= 0) {
+      s0 = h->grain_node_begin[id.yshift_grain] - ih.node_begin;
+      s1 = h->grain_node_begin[id.yshift_grain + 1] - ih.node_begin;
+    }
     for (size_t e = 0; e < n; ++e) {
       const size_t a = at(e);
-      for (int k = 0; k < D; ++k)
-        dst[e * D + k] = field == PG_FIELD_CURRPOS ? el(a, k) + el(a, D + k) : el(a, D + k);
+      for (int k = 0; k < D; ++k) {
+        double X = el(a, k);
+        if (k == 1 && e >= s0 && e < s1) X = X + id.yshift;
+        dst[e * D + k] = field == PG_FIELD_CURRPOS ? X + el(a, D + k) : el(a, D + k);
+      }
     }
     return PG_OK;
   }
@@ -1386,9 +1465,11 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
   if (nsteps == 0 || checkpoint_every == 0) return fail(PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
   if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
   const InstDev &id = h->inst_dev[0];
-  if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped)
+  if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped ||
+      h->inst[0].reset_step != 0)
     return fail(PG_ERR_INVALID,
-                "the adjoint supports no torque, no timed movable/stoppable toggles and no clamped nodes");
+                "the adjoint supports no torque, no timed movable/stoppable toggles, no clamped "
+                "nodes and no reset_partzero_y");
   HIPCHK(hipSetDevice(h->device));
   auto &a = h->adj;
   RC(adj_alloc(h, nsteps, checkpoint_every));

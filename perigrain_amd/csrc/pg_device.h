@@ -146,6 +146,17 @@ __device__ __forceinline__ void rec_X(const StepArgs &A, uint32_t i, double *X) 
 #pragma unroll
   for (int k = 0; k < D; ++k) X[k] = A.rec[0][rec_index(A.ns, i, k)]; // X: both parities
 }
+// The reference position `pos` of node i of grain g: X, plus the y shift of
+// reset_partzero_y on the shifted grain (timeloop.h:646-648 adds it to pos in
+// place).  Positions (CurrPos, contact, walls, boxes, the self-contact
+// reference distance) use it; bonds keep X, since the reference does not
+// regenerate xi after the shift.
+template <int D>
+__device__ __forceinline__ void pos_X(const StepArgs &A, const InstDev &ip, uint32_t g,
+                                      uint32_t i, double *X) {
+  rec_X<D>(A, i, X);
+  if ((int32_t)g == ip.yshift_grain) X[1] = X[1] + ip.yshift;
+}
 template <int D>
 __device__ __forceinline__ void rec_u(const double *R, uint32_t ns, uint32_t i, double *u) {
 #pragma unroll

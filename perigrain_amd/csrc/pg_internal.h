@@ -93,8 +93,12 @@ struct InstDev {
   uint32_t node_begin, node_end;
   uint32_t ckey_begin, ckey_end; // contact-order keys of this instance
   int32_t nsides;
-  int32_t pad;
+  // reset_partzero_y (timeloop.h:629-647): once it happened, the positions
+  // `pos` of this packed grain (the instance's first) are X with yshift added
+  // to the y component; -1 before.  Bonds keep X.
+  int32_t yshift_grain;
   double skin_half2; // rebuild when a contact node moved more than this (squared)
+  double yshift;
 };
 
 // Cell grid of one instance for the Verlet-list build.

@@ -14,6 +14,9 @@ hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int 
                        hipStream_t s);
 hipError_t set_step_lds_limit(int dim, uint32_t bytes);
 hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s);
+// Grain boxes of step t from its positions, into a cleared buffer.
+hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long long *keys,
+                        hipStream_t s);
 hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gmean,
                              hipStream_t s);
 hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,

@@ -369,7 +369,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], ui[D], xc[D];
     rec_u<D>(U, A.ns, i, ui);
-    rec_X<D>(A, i, Xc);
+    pos_X<D>(A, ip, g, i, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
     const uint8_t info = A.node_info[i];
@@ -419,7 +419,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
             if (c + 1 < c1) nxt = A.cl_fast[c + 1];
           }
           double Xj[D], uj[D], vj[D];
-          rec_X<D>(A, j, Xj);
+          pos_X<D>(A, ip, gj, j, Xj);
           rec_u<D>(U, A.ns, j, uj);
           load_vec<D>(vprev, ns, j, vj);
           const double Vj = A.V[j];
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], un[D], u0[D];
-    rec_X<D>(A, i, Xc);
+    pos_X<D>(A, ip, g, i, Xc);
     rec_u<D>(A.rec[0], A.ns, i, u0);
     const bool mov = grain_movable(gp, ip, g, 1);
 #pragma unroll
@@ -622,6 +622,30 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
   wave_box_atomic<D>(A.bbox[1], g, valid, xn);
 }
 
+// Grain boxes of step t (get_minmax, contact.h:1023-1062) from the positions
+// of step t, into a cleared box buffer: after reset_partzero_y moves a grain
+// between the step that computed them and the step that uses them.
+template <int D>
+__global__ __launch_bounds__(256) void k_boxes(const StepArgs A, const uint32_t t,
+                                               unsigned long long *keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < A.n_nodes;
+  uint32_t g = 0xFFFFFFFFu;
+  double x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = 0.0;
+  if (valid) {
+    g = A.node_grain[i];
+    const InstDev &ip = A.insts[A.grains[g].inst];
+    double Xc[D], u[D];
+    pos_X<D>(A, ip, g, i, Xc);
+    rec_u<D>(A.rec[t & 1], A.ns, i, u);
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = Xc[k] + u[k];
+  }
+  wave_box_atomic<D>(keys, g, valid, x);
+}
+
 // mean_CurrPos of every grain (particle2.h:89-98), summed sequentially in the
 // reference node order.
 template <int D>
@@ -630,6 +654,7 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.n_grains) return;
   const GrainDev gp = A.grains[g];
+  const InstDev &ipg = A.insts[gp.inst];
   const double *U = A.rec[t & 1];
   double m[D];
 #pragma unroll
@@ -637,7 +662,7 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
   for (uint32_t e = gp.node_begin; e < gp.node_end; ++e) {
     const uint32_t n = A.ext2int[e];
     double Xc[D], un[D];
-    rec_X<D>(A, n, Xc);
+    pos_X<D>(A, ipg, g, n, Xc);
     rec_u<D>(U, A.ns, n, un);
 #pragma unroll
     for (int k = 0; k < D; ++k) m[k] += (Xc[k] + un[k]);
@@ -715,9 +740,10 @@ template <int D>
 __device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_t i,
                                           double *x, uint32_t &inst) {
   if (!(A.node_info[i] & NI_CONTACT)) return false;
-  const GrainDev gp = A.grains[A.node_grain[i]];
+  const uint32_t gi = A.node_grain[i];
+  const GrainDev gp = A.grains[gi];
   double Xc[D], un[D];
-  rec_X<D>(A, i, Xc);
+  pos_X<D>(A, A.insts[gp.inst], gi, i, Xc);
   rec_u<D>(A.rec[t & 1], A.ns, i, un);
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = Xc[k] + un[k];
@@ -902,7 +928,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
   if (mov && (info & NI_CONTACT) && (selfc || stop)) {
     const double *U = A.rec[t & 1];
     double Xc[D], xc[D], box[2 * D], ui[D];
-    rec_X<D>(A, i, Xc);
+    pos_X<D>(A, ip, g, i, Xc);
     rec_u<D>(U, A.ns, i, ui);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k];
@@ -911,7 +937,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
     for (uint32_t c = A.cl_ptr[i]; c < A.cl_ptr[i + 1]; ++c) {
       const uint32_t j = A.cl_j[c];
       double Xj[D], uj[D], d[D], r, R, K;
-      rec_X<D>(A, j, Xj);
+      pos_X<D>(A, ip, A.node_grain[j], j, Xj);
       rec_u<D>(U, A.ns, j, uj);
       if (!contact_gate<D>(A, t, A.node_grain[j], A.cl_slot[c], g, gp, ip, Xc, Xj, selfc, stop,
                            box, R, K) ||
@@ -998,6 +1024,14 @@ hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s) {
   const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
   if (dim == 2) hipLaunchKernelGGL(k_prologue<2>, dim3(nb), dim3(256), 0, s, A);
   else hipLaunchKernelGGL(k_prologue<3>, dim3(nb), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long long *keys,
+                        hipStream_t s) {
+  const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
+  if (dim == 2) hipLaunchKernelGGL(k_boxes<2>, dim3(nb), dim3(256), 0, s, A, t, keys);
+  else hipLaunchKernelGGL(k_boxes<3>, dim3(nb), dim3(256), 0, s, A, t, keys);
   return hipGetLastError();
 }
 

@@ -115,6 +115,45 @@ def test_3d_compression_with_walls_bit_exact(oracle_threads, monkeypatch, lds):
     assert np.any(r != 0), "top wall should push"
 
 
+def _falling_disk(seed, reset_step):
+    # disk 0 starts 4R above disk 1 and falls at 50 m/s; at reset_step
+    # reset_partzero_y moves it (pos shifted in place, bonds unchanged) to
+    # sit 0.9R + R_c above disk 1's top, so it lands on disk 1
+    s, p = setups.two_disk_collision(seed=seed, gap=5e-5)
+    a = int(s.node_offset[1])
+    s.pos[:a] += np.array([s.pos[a:, 0].mean() - s.pos[:a, 0].mean(), 4e-3])
+    s.vel[:] = 0.0
+    s.vel[:a] = [0.0, -50.0]
+    p.reset_partzero_y, p.reset_partzero_y_timestep, p.wheel_rad = 1, reset_step, 0.9e-3
+    return s, p
+
+
+def test_reset_partzero_y_bit_exact(oracle_threads):
+    s, p = _falling_disk(3, 30)
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (29, 30, 31, 400):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    assert len(o.contacts()) > 0, "the moved disk should touch the other"
+
+
+def test_reset_partzero_y_batched_equals_single():
+    sp = [_falling_disk(3, 30), _falling_disk(4, 45)]
+    ss = [s for s, _ in sp]
+    ps = [p for _, p in sp]
+    with Stepper(ss, ps, device=0) as b:
+        b.step(300)
+        for k, (s, p) in enumerate(sp):
+            with Stepper(s, p, device=0) as one:
+                one.step(25)
+                one.step(275)
+                for f in FIELDS:
+                    assert np.array_equal(b.read(f, inst=k), one.read(f)), f"instance {k} {f}"
+                assert np.array_equal(b.contacts(inst=k), one.contacts())
+
+
 def test_2d_settle_with_walls_close(oracle_threads):
     s, p = setups.bulk_settle_2d(seed=6, n_grains=16, box=(8e-3, 16e-3), nodes_per_grain=80)
     # start the grains at 100 m/s in random directions so that they collide

@@ -418,6 +418,42 @@ def test_set_movable_toggle():
     assert o.read("CurrPos")[0, 0] > 0.0
 
 
+def test_reset_partzero_y_moves_first_grain_pos():
+    """reset_partzero_y (timeloop.h:629-647) at step T: from pos + disp of step
+    T-1 (disp as the position update of T-1 left it), the largest y over the
+    other grains -- starting from pos of node 0 plus disp of node 1 of the
+    second grain -- plus wheel_rad plus the contact radius is where the first
+    grain's mean y goes; the difference is added to its pos y in place, so
+    CurrPos of step T is (pos_y + shift) + disp_y, and x is untouched."""
+    s, p = setups.two_disk_collision(seed=3, gap=5e-5)
+    T, wheel = 30, 0.9e-3
+    p.reset_partzero_y, p.reset_partzero_y_timestep, p.wheel_rad = 1, T, wheel
+    o = oracle.Oracle(s, p)
+    o.step(T - 1)
+    d0 = o.read("disp")
+    assert np.array_equal(o.read("CurrPos"), s.pos + d0), "no shift before step T"
+    a, b = int(s.node_offset[1]), int(s.node_offset[2])
+    top = float(s.pos[a, 1]) + float(d0[a + 1, 1])
+    for j in range(a, b):
+        v = float(s.pos[j, 1]) + float(d0[j, 1])
+        if v > top:
+            top = v
+    mean = 0.0
+    for j in range(0, a):
+        mean += float(s.pos[j, 1]) + float(d0[j, 1])
+    mean = mean / a
+    shift = ((top + wheel) + s.contact_radius) - mean
+    o.step(1)
+    d1 = o.read("disp")
+    x = o.read("CurrPos")
+    moved = s.pos[:a].copy()
+    moved[:, 1] = moved[:, 1] + shift
+    assert np.array_equal(x[:a], moved + d1[:a])
+    assert np.array_equal(x[a:], s.pos[a:] + d1[a:])
+    # the first grain now sits about wheel_rad + R_c above the others' top
+    assert abs(np.mean(x[:a, 1]) - (top + wheel + s.contact_radius)) < 1e-6
+
+
 def test_torque_is_tangential():
     """Torque density tau on a ring: each node's force is tau along the
     counter-clockwise tangent."""
