@@ -165,10 +165,14 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
       if (info & NI_CONTACT) {
         const uint32_t c0 = A.cl_ptr[i], c1 = A.cl_ptr[i + 1];
         for (uint32_t cc = c0; cc < c1; ++cc) {
+          // a same-grain pair still bonded at step t has no contact in either
+          // direction (its two slots break in the same step): skip it before
+          // loading the partner
+          const int32_t bs = A.cl_slot[cc];
+          if (bs >= 0 && A.break_step[bs] > t) continue;
           const uint32_t j = A.cl_j[cc];
           const uint32_t gj = A.node_grain[j];
           const GrainDev gpj = A.grains[gj];
-          const int32_t bs = A.cl_slot[cc];
           double Xj[D], uj[D], xj[D], vj[D];
           rec_X<D>(A, j, Xj);
           rec_u<D>(B.su, A.ns, j, uj);

@@ -810,7 +810,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->violation, 1));
   RC(dalloc(h, h->tbase, 1));
   RC(dalloc(h, h->wterm, (size_t)(CK ? CK : 1) * 24));
-  RC(dalloc(h, h->wflag, CK ? CK : 1));
+  RC(dalloc(h, h->wflag, (size_t)CK + kWflagSpare)); // zero spare: k_reaction reads whole chunks
   RC(dalloc(h, h->reaction, (size_t)n * 18));
   RC(dalloc(h, h->grids, n));
   RC(dalloc(h, h->ibox, (size_t)n * 2 * D));
@@ -870,7 +870,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   HIPCHK(hipMemsetAsync(h->force_out, 0, ND * sizeof(double), h->stream));
   HIPCHK(hipMemsetAsync(h->xb, 0, ND * sizeof(double), h->stream));
   HIPCHK(hipMemsetAsync(h->reaction, 0, (size_t)n * 18 * sizeof(double), h->stream));
-  HIPCHK(hipMemsetAsync(h->wflag, 0, CK ? CK : 1, h->stream));
+  HIPCHK(hipMemsetAsync(h->wflag, 0, (size_t)CK + kWflagSpare, h->stream));
   HIPCHK(hipMemsetAsync(h->wterm, 0, (size_t)(CK ? CK : 1) * 24 * sizeof(double), h->stream));
   HIPCHK(hipMemsetAsync(h->cl_ptr, 0, ((size_t)N + 1) * sizeof(uint32_t), h->stream));
   HIPCHK(hipMemsetAsync(h->gmean, 0, (h->any_torque ? (size_t)G * 3 : 1) * sizeof(double), h->stream));
@@ -1140,10 +1140,22 @@ int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
   while (h->t < end) {
     for (uint32_t t = h->t + 1; t <= end; ++t) {
       const size_t q = t - c - 1;
-      RC(d2d(h, a.su + q * rec_len(h), h->rec[t & 1], rec_len(h)));
-      RC(d2d(h, a.sv + q * vec_len(h), h->vel[(t + 1) & 1], vec_len(h)));
-      RC(d2d(h, a.swall + q * h->I * 6, h->wallpos[t & 1], (size_t)h->I * 6));
-      RC(d2d(h, a.sbox + q * box_len(h), h->bbox[t % 3], box_len(h)));
+      // snapshot of what step t reads: u^t, v^{t-1}, walls and boxes
+      CopySet cs;
+      auto w8 = [](const void *p) { return (unsigned long long *)const_cast<void *>(p); };
+      cs.dst[0] = w8(a.su + q * rec_len(h));
+      cs.src[0] = w8(h->rec[t & 1]);
+      cs.n[0] = rec_len(h);
+      cs.dst[1] = w8(a.sv + q * vec_len(h));
+      cs.src[1] = w8(h->vel[(t + 1) & 1]);
+      cs.n[1] = vec_len(h);
+      cs.dst[2] = w8(a.swall + q * h->I * 6);
+      cs.src[2] = w8(h->wallpos[t & 1]);
+      cs.n[2] = (size_t)h->I * 6;
+      cs.dst[3] = w8(a.sbox + q * box_len(h));
+      cs.src[3] = w8(h->bbox[t % 3]);
+      cs.n[3] = box_len(h);
+      HIPCHK(launch_copy4(cs, h->stream));
       HIPCHK(launch_step(h->dim, false, h->A, t, h->block, h->stream));
     }
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,

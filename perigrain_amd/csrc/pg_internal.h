@@ -184,6 +184,10 @@ struct AdjArgs {
   double *gpart;               // [blocks][3]: d/d(c scale), d/dKn, d/dmu partial sums
 };
 
+// Zero bytes after the last wall flag: the wall-reaction kernel reads the
+// flags in whole chunks of this many keys.
+constexpr size_t kWflagSpare = 4096;
+
 // Order-preserving map of an IEEE double onto uint64 (for atomic min/max).
 __host__ __device__ inline unsigned long long dkey(double x) {
   union {

@@ -14,6 +14,13 @@ hipError_t launch_step(int dim, bool record, const StepArgs &A, uint32_t t, int 
                        hipStream_t s);
 hipError_t set_step_lds_limit(int dim, uint32_t bytes);
 hipError_t launch_prologue(int dim, const StepArgs &A, hipStream_t s);
+// Up to four copies of 8-byte words (unused entries: n = 0) in one launch.
+struct CopySet {
+  unsigned long long *dst[4];
+  const unsigned long long *src[4];
+  size_t n[4];
+};
+hipError_t launch_copy4(const CopySet &cs, hipStream_t s);
 // Grain boxes of step t from its positions, into a cleared buffer.
 hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long long *keys,
                         hipStream_t s);

@@ -678,55 +678,74 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
 template <int D>
 __global__ __launch_bounds__(256) void k_reaction(const InstDev *insts, const double *wterm,
                                                   const uint8_t *wflag, double *reaction) {
-  constexpr int CH = 4096, PER = CH / 256;
+  constexpr int CH = (int)kWflagSpare, PER = CH / 256;
   __shared__ uint32_t list[CH];
-  __shared__ uint32_t wsum[256];
+  __shared__ uint32_t wtot[4];
   const InstDev ip = insts[blockIdx.x];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ns = ip.nsides;
   double acc = 0.0;
   const int s = tid / D, c = tid % D;
   const int dir = (D == 2) ? ((s < 2) ? 0 : 1) : (s % 3);
   for (uint32_t base = ip.ckey_begin; base < ip.ckey_end; base += CH) {
-    uint32_t cnt = 0;
+    // the keys of this chunk with a wall term, compacted in key order: each
+    // thread takes PER consecutive keys; a block-wide exclusive scan of the
+    // per-thread counts (wavefront shuffles, then the four wavefront totals)
+    // gives each thread its first output position
     uint32_t loc[PER];
+    uint32_t cnt = 0;
+    // (wflag has a chunk of zero bytes past its end, so every load is in
+    // bounds and none waits for a branch)
+    uint8_t f[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) f[q] = wflag[base + tid * PER + q];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const uint32_t key = base + tid * PER + q;
-      const bool on = key < ip.ckey_end && wflag[key];
+      const bool on = key < ip.ckey_end && f[q];
       loc[q] = on ? key : 0xFFFFFFFFu;
       cnt += on ? 1 : 0;
     }
-    wsum[tid] = cnt;
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t run = 0;
-      for (int q = 0; q < 256; ++q) {
-        const uint32_t v = wsum[q];
-        wsum[q] = run;
-        run += v;
-      }
-      list[CH - 1] = run; // total, read back below before being overwritten
+    uint32_t v = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(v, off, 64);
+      if (lane >= off) v += u;
     }
+    if (lane == 63) wtot[wave] = v;
     __syncthreads();
-    const uint32_t total = list[CH - 1];
-    __syncthreads();
-    uint32_t w = wsum[tid];
+    uint32_t w = v - cnt, total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < wave) w += wtot[q];
+      total += wtot[q];
+    }
 #pragma unroll
     for (int q = 0; q < PER; ++q)
       if (loc[q] != 0xFFFFFFFFu) list[w++] = loc[q];
     __syncthreads();
+    // one thread per (side, component) sums its terms in key order: terms
+    // 0 and 1 along the side's direction, term 2 or 3 across it.  Eight
+    // keys' terms are loaded together, then subtracted in order (a single
+    // term is followed by subtracting +0.0, which leaves every value as it is).
     if (tid < ns * D) {
-      for (uint32_t q = 0; q < total; ++q) {
-        const double *p = wterm + ((size_t)list[q] * 6 + s) * 4;
-        if (c == dir) {
-          acc -= p[0];
-          acc -= p[1];
-        } else if (c == (dir + 1) % D) {
-          acc -= p[2];
-        } else {
-          acc -= p[3];
+      const int o1 = (c == dir) ? 0 : ((c == (dir + 1) % D) ? 2 : 3);
+      const bool two = (c == dir);
+      for (uint32_t q = 0; q < total; q += 8) {
+        double a1[8], a2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t key = list[(q + k < total) ? q + k : q];
+          const double *p = wterm + ((size_t)key * 6 + s) * 4;
+          a1[k] = p[o1];
+          a2[k] = p[1];
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (q + k < total) {
+            acc = acc - a1[k];
+            acc = acc - (two ? a2[k] : 0.0);
+          }
       }
     }
     __syncthreads();
@@ -951,6 +970,25 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
     }
   }
   if (!FILL) cnt[i] = n;
+}
+
+// Up to four device-to-device copies of 8-byte words in one launch (the
+// per-step snapshots of a differentiable run's recompute).
+__global__ void k_copy4(CopySet cs) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    for (size_t i = i0; i < cs.n[r]; i += stride) cs.dst[r][i] = cs.src[r][i];
+}
+
+hipError_t launch_copy4(const CopySet &cs, hipStream_t s) {
+  size_t n = 0;
+  for (int r = 0; r < 4; ++r) n = cs.n[r] > n ? cs.n[r] : n;
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = (uint32_t)std::min<size_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_copy4, dim3(nb), dim3(256), 0, s, cs);
+  return hipGetLastError();
 }
 
 __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long a,
