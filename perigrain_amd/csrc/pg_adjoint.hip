@@ -120,40 +120,59 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
       const double Vi = A.V[i];
       const double c = gp.cnot;
 
-      // ---- bonds: F_i = sum_j c V_j s e,  dF_i/du_j = c V_j M,  M symmetric
+      // ---- bonds: F_i = sum_j c V_j s e,  dF_i/du_j = c V_j M,  M symmetric.
+      // Slots are taken four at a time (one 16-byte load of column words and
+      // one of break steps); the four partners' data are loaded together.
+      // Padding slots and bonds removed after an earlier step read the ghost
+      // (finite values) and their terms are selected away.
       {
         const uint32_t sl = i >> 6, lane = i & 63;
         const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-        for (uint32_t k = 0; k < width; ++k) {
-          const uint64_t e_at = bslot(base, lane, k);
-          const uint32_t cw = A.bcol[e_at];
-          if (cw == BCOL_PAD) break;
-          if (A.break_step[e_at] < t) continue; // removed after an earlier step
-          const uint32_t j = cw & BCOL_INDEX;
-          double Xj[D], uj[D], lFj[D], xb[D], eta[D], e[D], q[D];
-          rec_X<D>(A, j, Xj);
-          rec_u<D>(B.su, A.ns, j, uj);
-          load_cols<D>(B.lf_in, ns, j, lFj);
+        const uint32_t *bc = A.bcol + base + BGROUP * lane;
+        const uint32_t *bk = A.break_step + base + BGROUP * lane;
+        for (uint32_t k0 = 0; k0 < width; k0 += BGROUP) {
+          const uint4 cw4 = *reinterpret_cast<const uint4 *>(bc + SLICE * k0);
+          const uint4 bk4 = *reinterpret_cast<const uint4 *>(bk + SLICE * k0);
+          const uint32_t cws[4] = {cw4.x, cw4.y, cw4.z, cw4.w};
+          const uint32_t bks[4] = {bk4.x, bk4.y, bk4.z, bk4.w};
+          double Xj[4][D], uj[4][D], lFj[4][D], Vj[4];
+          bool live[4];
 #pragma unroll
-          for (int r = 0; r < D; ++r) {
-            xb[r] = Xj[r] - Xi[r];
-            eta[r] = (uj[r] - ui[r]) + xb[r];
-          }
-          const double xn = sqrt(red<D>(xb, xb));
-          const double n = sqrt(red<D>(eta, eta));
-          const double s = (n - xn) / xn;
-          const double Vj = A.V[j];
-          double eq = 0.0, el = 0.0;
-#pragma unroll
-          for (int r = 0; r < D; ++r) {
-            e[r] = eta[r] / n;
-            q[r] = c * Vi * lFj[r] - c * Vj * lF[r];
-            eq += e[r] * q[r];
-            el += e[r] * lF[r];
+          for (int m = 0; m < 4; ++m) {
+            live[m] = cws[m] != BCOL_PAD && bks[m] >= t; // a bond broken at t is still used at t
+            const uint32_t j = live[m] ? (cws[m] & BCOL_INDEX) : A.ghost;
+            rec_X<D>(A, j, Xj[m]);
+            rec_u<D>(B.su, A.ns, j, uj[m]);
+            load_cols<D>(B.lf_in, ns, j, lFj[m]);
+            Vj[m] = A.V[j];
           }
 #pragma unroll
-          for (int r = 0; r < D; ++r) au[r] += e[r] * eq / xn + s * (q[r] - e[r] * eq) / n;
-          gc += c * Vj * s * el;
+          for (int m = 0; m < 4; ++m) {
+            double xb[D], eta[D], e[D], q[D];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+              xb[r] = Xj[m][r] - Xi[r];
+              eta[r] = (uj[m][r] - ui[r]) + xb[r];
+            }
+            const double xn = sqrt(red<D>(xb, xb));
+            const double n = sqrt(red<D>(eta, eta));
+            const double s = (n - xn) / xn;
+            double eq = 0.0, el = 0.0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+              e[r] = eta[r] / n;
+              q[r] = c * Vi * lFj[m][r] - c * Vj[m] * lF[r];
+              eq += e[r] * q[r];
+              el += e[r] * lF[r];
+            }
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+              const double du = e[r] * eq / xn + s * (q[r] - e[r] * eq) / n;
+              au[r] += live[m] ? du : 0.0;
+            }
+            const double dc = c * Vj[m] * s * el;
+            gc += live[m] ? dc : 0.0;
+          }
         }
       }
 

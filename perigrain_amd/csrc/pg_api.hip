@@ -127,6 +127,10 @@ struct pg_handle {
   bool any_clamped = false;
   double *hist = nullptr; // while recording: [steps][insts][18] wall reaction of every step
   uint32_t hist_t0 = 0;
+  // while recording: wall terms and flags of the last check_every steps, one
+  // ring slot per step, so that one launch sums a whole chunk's reactions
+  double *wring = nullptr;
+  uint8_t *fring = nullptr;
   struct Adj {
     bool recorded = false;
     uint32_t t0 = 0, T = 0, K = 0, nck = 0;
@@ -931,6 +935,8 @@ int ensure_graph(pg_handle *h) {
 size_t rec_len(const pg_handle *h) { return (size_t)h->ns * 2 * h->dim; }
 size_t vec_len(const pg_handle *h) { return (size_t)h->ns * h->dim; }
 size_t box_len(const pg_handle *h) { return (size_t)h->G * 2 * h->dim; }
+size_t wring_stride(const pg_handle *h) { return (size_t)(h->CK ? h->CK : 1) * 24; }
+size_t fring_stride(const pg_handle *h) { return (size_t)h->CK + kWflagSpare; }
 
 // reset_partzero_y of instance s before step tr (timeloop.h:629-647): the
 // largest y among the other grains' nodes of pos + disp of step tr-1
@@ -1009,10 +1015,15 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     } else {
       for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
         if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
-        HIPCHK(launch_step(h->dim, s == t_end || h->hist, h->A, s, h->block, h->stream));
-        if (h->hist) // wall reaction of every step (differentiable runs)
-          HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag,
-                                 h->hist + (size_t)(s - h->hist_t0 - 1) * h->I * 18, h->stream));
+        if (h->hist) { // recording: the step's wall terms go to its ring slot
+          StepArgs As = h->A;
+          const uint32_t slot = (s - 1) % h->check_every;
+          As.wterm = h->wring + (size_t)slot * wring_stride(h);
+          As.wflag = h->fring + (size_t)slot * fring_stride(h);
+          HIPCHK(launch_step(h->dim, true, As, s, h->block, h->stream));
+        } else {
+          HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+        }
       }
     }
     HIPCHK(hipEventRecord(h->kev[1], h->stream));
@@ -1020,6 +1031,13 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
                           h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     const uint32_t v = h->h_pin[0];
+    if (h->hist) { // wall reactions of the steps that ran, in one launch
+      const uint32_t ran = v > h->t ? std::min<uint32_t>(v - 1, h->t + chunk) - h->t : 0;
+      HIPCHK(launch_reactions(h->dim, h->I, h->insts, h->wring, h->fring,
+                              h->hist + (size_t)(h->t - h->hist_t0) * h->I * 18, ran,
+                              h->t % h->check_every, h->check_every, wring_stride(h),
+                              fring_stride(h), (size_t)h->I * 18, h->stream));
+    }
     if (v > h->t + chunk) { // no launch of this run returned early
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
@@ -1035,7 +1053,12 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     }
     h->t += chunk;
   }
-  HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag, h->reaction, h->stream));
+  if (h->hist) // the last step's reaction is the history's last entry
+    HIPCHK(hipMemcpyAsync(h->reaction, h->hist + (size_t)(t_end - h->hist_t0 - 1) * h->I * 18,
+                          (size_t)h->I * 18 * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+  else
+    HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag, h->reaction, h->stream));
   return PG_OK;
 }
 
@@ -1485,6 +1508,11 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
   HIPCHK(hipSetDevice(h->device));
   auto &a = h->adj;
   RC(adj_alloc(h, nsteps, checkpoint_every));
+  if (!h->wring) {
+    RC(dalloc(h, h->wring, (size_t)h->check_every * wring_stride(h)));
+    RC(dalloc(h, h->fring, (size_t)h->check_every * fring_stride(h)));
+    HIPCHK(hipMemsetAsync(h->fring, 0, (size_t)h->check_every * fring_stride(h), h->stream));
+  }
   HIPCHK(launch_adj_init_break(h->d_bcol, a.break_step, h->E, h->stream));
   a.t0 = h->t;
   a.T = nsteps;

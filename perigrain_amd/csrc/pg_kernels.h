@@ -29,6 +29,13 @@ hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gme
 hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
                            const double *wterm, const uint8_t *wflag, double *reaction,
                            hipStream_t s);
+// Reactions of nsteps recorded steps at once: step y reads ring slot
+// (first + y) % slots of the wall terms (wstride apart) and flags (fstride
+// apart) and writes reaction + y * rstride.
+hipError_t launch_reactions(int dim, uint32_t n_insts, const InstDev *insts,
+                            const double *wterm, const uint8_t *wflag, double *reaction,
+                            uint32_t nsteps, uint32_t first, uint32_t slots, size_t wstride,
+                            size_t fstride, size_t rstride, hipStream_t s);
 hipError_t launch_fill_box(unsigned long long *p, size_t n, int dim, hipStream_t s);
 hipError_t launch_inst_box(int dim, const StepArgs &A, uint32_t t, unsigned long long *ibox,
                            hipStream_t s);

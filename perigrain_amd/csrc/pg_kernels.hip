@@ -672,12 +672,21 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
   for (int k = 0; k < D; ++k) gmean[(size_t)g * 3 + k] = m[k] / (double)nn;
 }
 
-// Wall reaction of a recorded step, summed in the reference's sequential
-// order (grain, node in contact order, then side): one block per instance.
-// contact.h:339,374,411-412,428 via timeloop.h:753-769.
+// Wall reaction of recorded steps, summed in the reference's sequential
+// order (grain, node in contact order, then side): one block per instance
+// (x) and step (y).  Step y's wall terms and flags are in ring slot
+// (first + y) % slots, wstride/fstride apart; its reaction goes to
+// reaction + y * rstride.  contact.h:339,374,411-412,428 via
+// timeloop.h:753-769.
 template <int D>
 __global__ __launch_bounds__(256) void k_reaction(const InstDev *insts, const double *wterm,
-                                                  const uint8_t *wflag, double *reaction) {
+                                                  const uint8_t *wflag, double *reaction,
+                                                  uint32_t first, uint32_t slots, size_t wstride,
+                                                  size_t fstride, size_t rstride) {
+  const uint32_t slot = (first + blockIdx.y) % slots;
+  wterm += slot * wstride;
+  wflag += slot * fstride;
+  reaction += blockIdx.y * rstride;
   constexpr int CH = (int)kWflagSpare, PER = CH / 256;
   __shared__ uint32_t list[CH];
   __shared__ uint32_t wtot[4];
@@ -1084,12 +1093,22 @@ hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gme
 hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
                            const double *wterm, const uint8_t *wflag, double *reaction,
                            hipStream_t s) {
+  return launch_reactions(dim, n_insts, insts, wterm, wflag, reaction, 1, 0, 1, 0, 0,
+                          (size_t)n_insts * 18, s);
+}
+
+hipError_t launch_reactions(int dim, uint32_t n_insts, const InstDev *insts,
+                            const double *wterm, const uint8_t *wflag, double *reaction,
+                            uint32_t nsteps, uint32_t first, uint32_t slots, size_t wstride,
+                            size_t fstride, size_t rstride, hipStream_t s) {
+  if (nsteps == 0) return hipSuccess;
+  const dim3 g(n_insts, nsteps);
   if (dim == 2)
-    hipLaunchKernelGGL(k_reaction<2>, dim3(n_insts), dim3(256), 0, s, insts, wterm, wflag,
-                       reaction);
+    hipLaunchKernelGGL(k_reaction<2>, g, dim3(256), 0, s, insts, wterm, wflag, reaction, first,
+                       slots, wstride, fstride, rstride);
   else
-    hipLaunchKernelGGL(k_reaction<3>, dim3(n_insts), dim3(256), 0, s, insts, wterm, wflag,
-                       reaction);
+    hipLaunchKernelGGL(k_reaction<3>, g, dim3(256), 0, s, insts, wterm, wflag, reaction, first,
+                       slots, wstride, fstride, rstride);
   return hipGetLastError();
 }
 
