@@ -190,6 +190,25 @@ int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst,
 int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs,
                      uint64_t *pairs);
 
+/* On-device post-processing of the last completed step t, per grain of
+ * instance inst in the reference's grain order: the per-particle quantities
+ * of read_sim_output.py extract_bulk (:143-266) and the force sum of
+ * read_force_val (:287-316, used by gen_wall_reaction.py), without writing
+ * a snapshot.  out holds PG_GRAIN_STATS(dim) doubles per grain:
+ *   [0, dim)        sum of force density over the grain's nodes
+ *   [dim, 2dim)     mean velocity
+ *   [2dim, 3dim)    mean acceleration
+ *   [3dim, 4dim)    mean CurrPos
+ *   4dim            damage: mean over nodes of (bonds at creation - intact
+ *                   bonds) / bonds at creation (NaN for a node without bonds,
+ *                   as numpy gives)
+ *   [4dim+1, 5dim+1) sum of vol * rho * acc
+ * The sums are parallel reductions, so they agree with numpy's to rounding,
+ * not bit for bit.  Damage counts against the bonds the handle was created
+ * with (a run resumed from a snapshot starts from that snapshot's bonds). */
+#define PG_GRAIN_STATS(dim) (5 * (dim) + 1)
+int pg_grain_stats(pg_handle *h, int inst, double *out);
+
 /* Stream (hipStream_t) the handle enqueues its kernels on. */
 int pg_stream(pg_handle *h, void **stream);
 

@@ -131,6 +131,8 @@ struct pg_handle {
   // ring slot per step, so that one launch sums a whole chunk's reactions
   double *wring = nullptr;
   uint8_t *fring = nullptr;
+  double *d_stats = nullptr; // pg_grain_stats output
+  size_t stats_cap = 0;
   struct Adj {
     bool recorded = false;
     uint32_t t0 = 0, T = 0, K = 0, nck = 0;
@@ -1334,6 +1336,23 @@ int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
     for (int s = 0; s < PG_MAX_SIDES; ++s)
       for (int k = 0; k < D; ++k) reaction[s * D + k] = r[s * 3 + k];
   }
+  return PG_OK;
+}
+
+int pg_grain_stats(pg_handle *h, int inst, double *out) {
+  if (!h || !out || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+  const int D = h->dim;
+  const InstHost &ih = h->inst[inst];
+  const uint32_t ng = ih.grain_end - ih.grain_begin;
+  const size_t nv = (size_t)PG_GRAIN_STATS(D) * ng;
+  HIPCHK(hipSetDevice(h->device));
+  if (nv > h->stats_cap) {
+    RC(dfree(h, h->d_stats));
+    RC(dalloc(h, h->d_stats, nv));
+    h->stats_cap = nv;
+  }
+  HIPCHK(launch_grain_stats(D, h->A, h->t, ih.grain_begin, ng, h->d_stats, h->stream));
+  RC(d2h(h, out, h->d_stats, nv));
   return PG_OK;
 }
 

@@ -21,6 +21,9 @@ struct CopySet {
   size_t n[4];
 };
 hipError_t launch_copy4(const CopySet &cs, hipStream_t s);
+// Per-grain post-processing of step t for grains g0 .. g0+ng-1 (pg_grain_stats).
+hipError_t launch_grain_stats(int dim, const StepArgs &A, uint32_t t, uint32_t g0,
+                              uint32_t ng, double *out, hipStream_t s);
 // Grain boxes of step t from its positions, into a cleared buffer.
 hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long long *keys,
                         hipStream_t s);

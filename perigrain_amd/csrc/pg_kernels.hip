@@ -646,6 +646,70 @@ __global__ __launch_bounds__(256) void k_boxes(const StepArgs A, const uint32_t 
   wave_box_atomic<D>(keys, g, valid, x);
 }
 
+// Per-grain post-processing of the completed step t (the quantities of
+// read_sim_output.py extract_bulk and read_force_val): one block per grain,
+// each thread summing a strided share of the grain's nodes, then a block
+// reduction.  out[g - g0][5D+1] as documented at pg_grain_stats.
+template <int D>
+__global__ __launch_bounds__(256) void k_grain_stats(const StepArgs A, const uint32_t t,
+                                                     const uint32_t g0, double *out) {
+  constexpr int NV = 5 * D + 1;
+  __shared__ double red_s[NV][256];
+  const uint32_t g = g0 + blockIdx.x;
+  const GrainDev gp = A.grains[g];
+  const InstDev &ip = A.insts[gp.inst];
+  const uint32_t ns = A.ns;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (uint32_t e = gp.node_begin + threadIdx.x; e < gp.node_end; e += blockDim.x) {
+    const uint32_t n = A.ext2int[e];
+    double X[D], u[D];
+    pos_X<D>(A, ip, g, n, X);
+    rec_u<D>(A.rec[t & 1], ns, n, u);
+    const double vr = A.V[n] * gp.rho;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double a = A.acc[(size_t)k * ns + n];
+      acc[k] += A.force_out[(size_t)k * ns + n];
+      acc[D + k] += A.vel[t & 1][(size_t)k * ns + n];
+      acc[2 * D + k] += a;
+      acc[3 * D + k] += X[k] + u[k];
+      acc[4 * D + 1 + k] += vr * a;
+    }
+    // bonds at creation: the row's non-padding slots; intact: those without
+    // the broken flag
+    const uint32_t sl = n >> 6, lane = n & 63;
+    const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+    uint32_t orig = 0, now = 0;
+    for (uint32_t k = 0; k < width; ++k) {
+      const uint32_t w = A.bcol[bslot(base, lane, k)];
+      if (w == BCOL_PAD) break;
+      ++orig;
+      now += (w & BCOL_BROKEN) ? 0u : 1u;
+    }
+    acc[4 * D] += (double)(orig - now) / (double)orig;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) red_s[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) red_s[k][threadIdx.x] += red_s[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nn = (double)(gp.node_end - gp.node_begin);
+    double *o = out + (size_t)blockIdx.x * NV;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const bool mean = (k >= D && k < 4 * D + 1);
+      o[k] = mean ? red_s[k][0] / nn : red_s[k][0];
+    }
+  }
+}
+
 // mean_CurrPos of every grain (particle2.h:89-98), summed sequentially in the
 // reference node order.
 template <int D>
@@ -1079,6 +1143,14 @@ hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long lo
   const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
   if (dim == 2) hipLaunchKernelGGL(k_boxes<2>, dim3(nb), dim3(256), 0, s, A, t, keys);
   else hipLaunchKernelGGL(k_boxes<3>, dim3(nb), dim3(256), 0, s, A, t, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_grain_stats(int dim, const StepArgs &A, uint32_t t, uint32_t g0,
+                              uint32_t ng, double *out, hipStream_t s) {
+  if (ng == 0) return hipSuccess;
+  if (dim == 2) hipLaunchKernelGGL(k_grain_stats<2>, dim3(ng), dim3(256), 0, s, A, t, g0, out);
+  else hipLaunchKernelGGL(k_grain_stats<3>, dim3(ng), dim3(256), 0, s, A, t, g0, out);
   return hipGetLastError();
 }
 

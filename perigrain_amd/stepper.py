@@ -97,6 +97,19 @@ class Stepper:
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return out
 
+    def grain_stats(self, inst=0):
+        """Per-grain post-processing of the last completed step on the device
+        (pg_grain_stats; read_sim_output.py extract_bulk / read_force_val):
+        arrays over the instance's grains in reference order."""
+        _, ng, _ = self.sizes(inst)
+        d = self.dim
+        out = np.empty((ng, 5 * d + 1), dtype=np.float64)
+        check(lib().pg_grain_stats(self._h, inst,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return {"force_sum": out[:, :d], "mean_vel": out[:, d:2 * d],
+                "mean_acc": out[:, 2 * d:3 * d], "mean_CurrPos": out[:, 3 * d:4 * d],
+                "damage": out[:, 4 * d], "vol_rho_acc_sum": out[:, 4 * d + 1:]}
+
     def conn(self, grain, inst=0):
         rows = ctypes.c_uint64()
         check(lib().pg_read_conn(self._h, inst, grain, None, 0, ctypes.byref(rows)))
