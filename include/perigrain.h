@@ -103,6 +103,14 @@ typedef struct pg_params {
   uint32_t reset_partzero_y_timestep;
   double wheel_rad;
   double wall_speed[PG_MAX_SIDES]; /* same order as pg_setup.wall */
+  /* 1: the loop of run_timeloop_worm (timeloop.h:1017-1369) instead of
+   * run_timeloop: phases of 1000 steps (clamped front nodes 0 and 3 of every
+   * grain in every third phase, back nodes 1 and 2 otherwise; nodes 0-3
+   * never move in y), extforce only in the front-clamped phases, every grain
+   * advancing its position, contact not gated by stoppable; the timed
+   * toggles, the extforce ramp and torque do not exist in that loop.  Every
+   * grain needs at least four nodes; not with the adjoint entry points. */
+  int32_t timeloop_worm;
 } pg_params;
 
 /* Fields readable with pg_read_nodes. */

@@ -551,15 +551,20 @@ template <unsigned D, class T> struct Oracle : OracleBase {
   void one_step(uint32_t t, int threads, bool rec_contacts) {
     const unsigned G = (unsigned)PArr.size();
     zero_reaction(); // setzero_reaction
-    if (TL.set_movable_index != -1) {
+    // run_timeloop_worm (timeloop.h:1017-1369): the same step except for
+    // the lines marked "worm"; it has no timed toggles, ramp or torque
+    const bool worm = TL.timeloop_worm != 0;
+    // worm: clamp the front (0, 3) or the back (1, 2) nodes (timeloop.h:1106-1116)
+    const int t_num = ((t / 1000) % (2 + 1) == 0) ? 0 : 1;
+    if (!worm && TL.set_movable_index != -1) {
       unsigned pi = (unsigned)TL.set_movable_index, ts = (unsigned)TL.set_movable_timestep;
       if (ts == t && pi < G) PArr[pi].movable = 1;
     }
-    if (TL.set_stoppable_index != -1) {
+    if (!worm && TL.set_stoppable_index != -1) {
       unsigned pi = (unsigned)TL.set_stoppable_index, ts = (unsigned)TL.set_stoppable_timestep;
       if (ts == t && pi < G) PArr[pi].stoppable = 1;
     }
-    if (TL.reset_partzero_y && G > 1) {
+    if (!worm && TL.reset_partzero_y && G > 1) {
       if (TL.reset_partzero_y_timestep == t) {
         double max_bulk_y = sval(PArr[1].pos[0](1) + PArr[1].disp[1](1));
         for (unsigned i = 1; i < G; ++i)
@@ -576,7 +581,7 @@ template <unsigned D, class T> struct Oracle : OracleBase {
 #pragma omp parallel for num_threads(threads) if (threads > 1)
     for (unsigned i = 0; i < G; ++i) {
       Grain<D, T> &P = PArr[i];
-      if (P.movable) {
+      if (P.movable || worm) { // worm: every grain (timeloop.h:1120-1124)
         const double h = (dt * dt * 0.5);
         for (unsigned n = 0; n < P.nnodes; ++n) {
           V inc = (dt * P.vel[n]) + (h * P.acc[n]);
@@ -606,7 +611,7 @@ template <unsigned D, class T> struct Oracle : OracleBase {
               contact_by_boundary<D, T>(P, P, CN, dt, ft, true, pairs, node_off[i], node_off[i]);
           }
         }
-        if (P.stoppable) {
+        if (P.stoppable || worm) { // worm: contact not gated (timeloop.h:1160)
           const double *C = &bbox[i * 6];
           for (unsigned j = 0; j < G; j++) {
             if (j != i) {
@@ -619,14 +624,17 @@ template <unsigned D, class T> struct Oracle : OracleBase {
               wall_force(P, dt, ft, &terms[i]);
           }
         }
-        if (TL.gradient_extforce) {
+        if (worm) { // timeloop.h:1183
+          const double fac = (double)((t_num + 1) % 2);
+          for (unsigned n = 0; n < P.nnodes; ++n) ft[n] += fac * P.extforce[n];
+        } else if (TL.gradient_extforce) {
           double extf_sc = (double)t / (double)TL.extforce_maxstep;
           if (extf_sc > 1) extf_sc = 1;
           for (unsigned n = 0; n < P.nnodes; ++n) ft[n] += extf_sc * P.extforce[n];
         } else {
           for (unsigned n = 0; n < P.nnodes; ++n) ft[n] += P.extforce[n];
         }
-        if (TL.enable_torque) {
+        if (!worm && TL.enable_torque) {
           V c = P.mean_CurrPos();
           unsigned taxis = P.torque_axis;
           for (unsigned nn = 0; nn < P.nnodes; nn++) {
@@ -680,6 +688,18 @@ template <unsigned D, class T> struct Oracle : OracleBase {
           P.disp[cid] = V::zero();
           P.vel[cid] = V::zero();
           P.acc[cid] = V::zero();
+        }
+        if (worm) { // timeloop.h:1215-1245
+          const unsigned held[2][2] = {{0, 3}, {1, 2}};
+          for (unsigned k = 0; k < 2; ++k) {
+            const unsigned cid = held[t_num][k];
+            P.vel[cid] = V::zero();
+            P.acc[cid] = V::zero();
+          }
+          for (unsigned cid = 0; cid < 4; ++cid) {
+            P.vel[cid](1) = T(0.0);
+            P.acc[cid](1) = T(0.0);
+          }
         }
       }
     }

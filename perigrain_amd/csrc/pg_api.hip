@@ -466,7 +466,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
     ih.nsides = D == 2 ? 4 : 6;
     const pg_params &pp = params[s];
-    ih.reset_step = pp.reset_partzero_y ? pp.reset_partzero_y_timestep : 0;
+    ih.reset_step = (pp.reset_partzero_y && !pp.timeloop_worm) ? pp.reset_partzero_y_timestep : 0;
     ih.wheel_rad = pp.wheel_rad;
   }
   // record gathers use 32-bit byte offsets of 16 bytes per node
@@ -510,6 +510,13 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         r.X[D] = st.vol[q];
         node_grain[i] = gg;
         if (!bdry_only && ih.contact) node_info[i] |= NI_CONTACT;
+      }
+      if (pp.timeloop_worm) { // nodes 0, 3 (front) and 1, 2 (back) of each grain
+        if (b - a < 4) return fail(PG_ERR_INVALID, "timeloop_worm needs four nodes in every grain");
+        node_info[ih.node_begin + a + 0] |= NI_WORM_FRONT;
+        node_info[ih.node_begin + a + 3] |= NI_WORM_FRONT;
+        node_info[ih.node_begin + a + 1] |= NI_WORM_BACK;
+        node_info[ih.node_begin + a + 2] |= NI_WORM_BACK;
       }
       for (int64_t c = st.clamped_offset[g]; c < st.clamped_offset[g + 1]; ++c) {
         const uint32_t cn = st.clamped_nodes[c];
@@ -576,11 +583,14 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
                (pp.allow_damping ? IF_DAMPING : 0) | (pp.allow_friction ? IF_FRICTION : 0) |
                (pp.self_contact ? IF_SELF_CONTACT : 0) | (st.allow_wall ? IF_WALL : 0) |
                (pp.gradient_extforce ? IF_GRAD_EXT : 0) | (pp.enable_torque ? IF_TORQUE : 0);
+    // run_timeloop_worm has no extforce ramp, torque or timed toggles
+    const bool worm = pp.timeloop_worm != 0;
+    if (worm) id.flags = (id.flags & ~(IF_GRAD_EXT | IF_TORQUE)) | IF_WORM;
     id.extforce_maxstep = pp.extforce_maxstep;
-    id.set_movable_grain = (pp.set_movable_index >= 0 && pp.set_movable_timestep > 0)
+    id.set_movable_grain = (!worm && pp.set_movable_index >= 0 && pp.set_movable_timestep > 0)
                                ? (int32_t)(ih.grain_begin + pp.set_movable_index) : -1;
     id.set_movable_step = (uint32_t)std::max(pp.set_movable_timestep, 0);
-    id.set_stoppable_grain = (pp.set_stoppable_index >= 0 && pp.set_stoppable_timestep > 0)
+    id.set_stoppable_grain = (!worm && pp.set_stoppable_index >= 0 && pp.set_stoppable_timestep > 0)
                                  ? (int32_t)(ih.grain_begin + pp.set_stoppable_index) : -1;
     id.set_stoppable_step = (uint32_t)std::max(pp.set_stoppable_timestep, 0);
     id.grain_begin = ih.grain_begin;
@@ -594,7 +604,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     id.yshift = 0.0;
     const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
     id.skin_half2 = sh * sh;
-    if (pp.enable_torque) h->any_torque = true;
+    if (pp.enable_torque && !worm) h->any_torque = true;
     if (ih.contact) h->any_contact = true;
   }
   h->grain_node_begin[G] = (uint32_t)N;
@@ -1520,10 +1530,10 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
   if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
   const InstDev &id = h->inst_dev[0];
   if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped ||
-      h->inst[0].reset_step != 0)
+      h->inst[0].reset_step != 0 || (id.flags & IF_WORM))
     return fail(PG_ERR_INVALID,
                 "the adjoint supports no torque, no timed movable/stoppable toggles, no clamped "
-                "nodes and no reset_partzero_y");
+                "nodes, no reset_partzero_y and not the worm loop");
   HIPCHK(hipSetDevice(h->device));
   auto &a = h->adj;
   RC(adj_alloc(h, nsteps, checkpoint_every));

@@ -63,10 +63,13 @@ constexpr int32_t IF_SELF_CONTACT = 16;
 constexpr int32_t IF_WALL = 32;
 constexpr int32_t IF_GRAD_EXT = 64;
 constexpr int32_t IF_TORQUE = 128;
+constexpr int32_t IF_WORM = 256; // the loop of run_timeloop_worm (timeloop.h:1017-1369)
 
 // node info bits
 constexpr uint8_t NI_CLAMPED = 1;
 constexpr uint8_t NI_CONTACT = 2; // takes part in node-node and wall contact
+constexpr uint8_t NI_WORM_FRONT = 4; // node 0 or 3 of its grain (run_timeloop_worm)
+constexpr uint8_t NI_WORM_BACK = 8;  // node 1 or 2 of its grain
 
 struct GrainDev {
   double rho, cnot, snot;

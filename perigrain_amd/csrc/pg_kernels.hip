@@ -392,7 +392,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 
       // ---- (b) contact from the sorted Verlet list
       const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
-      const bool stop = grain_stoppable(gp, ip, g, t);
+      const bool stop = (ip.flags & IF_WORM) || grain_stoppable(gp, ip, g, t);
       double boxg[2 * D];
       load_box<D>(A.bbox[t % 3], g, boxg);
       if ((info & NI_CONTACT) && (selfc || stop)) {
@@ -458,7 +458,14 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
       // ---- external force (timeloop.h:776-786) and torque (:790-824)
       double ex[D];
       load_vec<D>(A.ext, ns, i, ex);
-      if (ip.flags & IF_GRAD_EXT) {
+      if (ip.flags & IF_WORM) {
+        // timeloop.h:1183: extforce times (t_num + 1) % 2, i.e. only in the
+        // phases with the front nodes clamped (adding 0*extforce leaves F)
+        if (((t / 1000u) % 3u) == 0u) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) F[k] += ex[k];
+        }
+      } else if (ip.flags & IF_GRAD_EXT) {
         double sc = (double)t / (double)ip.extforce_maxstep;
         if (sc > 1) sc = 1;
 #pragma unroll
@@ -506,13 +513,30 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
           ai[k] = 0.0;
         }
       }
+      if (ip.flags & IF_WORM) {
+        // timeloop.h:1106-1116, 1215-1245: in phases of 1000 steps the front
+        // nodes (0, 3) are held in every third phase, the back nodes (1, 2)
+        // in the others; nodes 0-3 never move in y
+        const bool front = ((t / 1000u) % 3u) == 0u;
+        if ((front && (info & NI_WORM_FRONT)) || (!front && (info & NI_WORM_BACK))) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            vi[k] = 0.0;
+            ai[k] = 0.0;
+          }
+        }
+        if (info & (NI_WORM_FRONT | NI_WORM_BACK)) {
+          vi[1] = 0.0;
+          ai[1] = 0.0;
+        }
+      }
       store_vec<D>(A.acc, ns, i, ai);
     }
     store_vec<D>(vcur, ns, i, vi);
 
     // ---- position update of step t+1 (timeloop.h:661-669)
     double un[D];
-    if (grain_movable(gp, ip, g, t + 1)) {
+    if ((ip.flags & IF_WORM) || grain_movable(gp, ip, g, t + 1)) { // worm: every grain
 #pragma unroll
       for (int k = 0; k < D; ++k) un[k] = ui[k] + ((ip.dt * vi[k]) + (ip.h * ai[k]));
     } else {
@@ -609,7 +633,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     double Xc[D], un[D], u0[D];
     pos_X<D>(A, ip, g, i, Xc);
     rec_u<D>(A.rec[0], A.ns, i, u0);
-    const bool mov = grain_movable(gp, ip, g, 1);
+    const bool mov = (ip.flags & IF_WORM) || grain_movable(gp, ip, g, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double v = A.vel[0][(size_t)k * ns + i];
@@ -1016,7 +1040,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
   const bool mov = grain_movable(gp, ip, g, t);
   const bool brk = (ip.flags & IF_FRACTURE) && (gp.flags & GF_BREAKABLE);
   const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
-  const bool stop = grain_stoppable(gp, ip, g, t);
+  const bool stop = (ip.flags & IF_WORM) || grain_stoppable(gp, ip, g, t);
   if (mov && (info & NI_CONTACT) && (selfc || stop)) {
     const double *U = A.rec[t & 1];
     double Xc[D], xc[D], box[2 * D], ui[D];

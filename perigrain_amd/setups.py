@@ -96,6 +96,7 @@ class Params:
     reset_partzero_y_timestep: int = 0
     wheel_rad: float = 0.0
     wall_speed: np.ndarray = field(default_factory=lambda: np.zeros(6))
+    timeloop_worm: int = 0
 
 
 # ----------------------------------------------------------------- materials

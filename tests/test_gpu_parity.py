@@ -154,6 +154,21 @@ def test_reset_partzero_y_batched_equals_single():
                 assert np.array_equal(b.contacts(inst=k), one.contacts())
 
 
+def test_worm_loop_bit_exact(oracle_threads):
+    # run_timeloop_worm across the phase change at step 1000 (front nodes
+    # held and extforce on before it, back nodes held and extforce off after)
+    s, p = setups.two_disk_collision(seed=3, gap=5e-5)
+    s.extforce[:] = [0.0, -9.81 * 1200.0]
+    p.timeloop_worm = 1
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (10, 999, 1000, 1001, 1100):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    assert len(o.contacts()) > 0, "the disks should be in contact"
+
+
 def test_2d_settle_with_walls_close(oracle_threads):
     s, p = setups.bulk_settle_2d(seed=6, n_grains=16, box=(8e-3, 16e-3), nodes_per_grain=80)
     # start the grains at 100 m/s in random directions so that they collide

@@ -454,6 +454,35 @@ def test_reset_partzero_y_moves_first_grain_pos():
     assert abs(np.mean(x[:a, 1]) - (top + wheel + s.contact_radius)) < 1e-6
 
 
+def test_worm_loop_phases():
+    """run_timeloop_worm (timeloop.h:1106-1245): nodes 0 and 3 of a grain are
+    held (v = a = 0) while (t // 1000) % 3 == 0, nodes 1 and 2 otherwise;
+    nodes 0-3 never move in y; extforce acts only in the front-held phases;
+    every grain advances its position."""
+    pos = np.array([[0.0, 0.0], [1e-3, 0.0], [2e-3, 0.0], [3e-3, 0.0], [4e-3, 0.0]])
+    s, p = build([grain(pos, rho=1000.0)], 2, vels=[np.array([1.0, 1.0])], timeloop_worm=1)
+    s.extforce[:] = [2.0, 4.0]
+    o = oracle.Oracle(s, p)
+    o.step(10)  # t = 10: front held, extforce on
+    v, a = o.read("vel"), o.read("acc")
+    assert np.all(v[[0, 3]] == 0) and np.all(a[[0, 3]] == 0)
+    assert np.all(v[[1, 2], 1] == 0) and np.all(a[[1, 2], 1] == 0)
+    assert np.array_equal(a[4], (1 / 1000.0) * np.array([2.0, 4.0]))
+    assert np.array_equal(a[[1, 2], 0], [(1 / 1000.0) * 2.0] * 2)
+    o.step(1000)  # t = 1010: back held, no extforce
+    v, a = o.read("vel"), o.read("acc")
+    assert np.all(v[[1, 2]] == 0) and np.all(a[[1, 2]] == 0)
+    assert np.all(v[[0, 3], 1] == 0) and np.all(a[[0, 3], 1] == 0)
+    assert np.all(a[[0, 3, 4]] == 0), "no extforce while the back nodes are held"
+    assert v[4, 1] > 1.0, "node 4 kept the velocity gained in the first phase"
+    # a non-movable grain still advances its position in the worm loop
+    s2, p2 = build([grain(pos)], 2, vels=[np.array([1.0, 0.0])], timeloop_worm=1)
+    s2.movable[:] = 0
+    o2 = oracle.Oracle(s2, p2)
+    o2.step(5)
+    assert o2.read("CurrPos")[4, 0] > pos[4, 0]
+
+
 def test_torque_is_tangential():
     """Torque density tau on a ring: each node's force is tau along the
     counter-clockwise tangent."""
