@@ -103,15 +103,21 @@ typedef struct pg_params {
   uint32_t reset_partzero_y_timestep;
   double wheel_rad;
   double wall_speed[PG_MAX_SIDES]; /* same order as pg_setup.wall */
-  /* 1: the loop of run_timeloop_worm (timeloop.h:1017-1369) instead of
-   * run_timeloop: phases of 1000 steps (clamped front nodes 0 and 3 of every
-   * grain in every third phase, back nodes 1 and 2 otherwise; nodes 0-3
-   * never move in y), extforce only in the front-clamped phases, every grain
-   * advancing its position, contact not gated by stoppable; the timed
-   * toggles, the extforce ramp and torque do not exist in that loop.  Every
-   * grain needs at least four nodes; not with the adjoint entry points. */
-  int32_t timeloop_worm;
+  /* Which of the reference's loops the steps follow (PG_TIMELOOP_*):
+   * run_timeloop (timeloop.h:500-1015), run_timeloop_worm (:1017-1369) or
+   * run_timeloop_compact (:335-498).  The worm and compact loops advance
+   * every grain's position, do not gate contact by stoppable, and have no
+   * timed toggles or torque.  The worm loop also runs phases of 1000 steps:
+   * the front nodes 0 and 3 of every grain are held in every third phase
+   * and extforce acts only then, the back nodes 1 and 2 are held in the
+   * others, and nodes 0-3 never move in y (every grain needs four nodes);
+   * it has no extforce ramp.  Neither works with the adjoint entry points. */
+  int32_t timeloop_variant;
 } pg_params;
+
+#define PG_TIMELOOP_STANDARD 0
+#define PG_TIMELOOP_WORM 1
+#define PG_TIMELOOP_COMPACT 2
 
 /* Fields readable with pg_read_nodes. */
 #define PG_FIELD_CURRPOS 0 /* CurrPos of the last completed step t */

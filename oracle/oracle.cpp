@@ -551,20 +551,22 @@ template <unsigned D, class T> struct Oracle : OracleBase {
   void one_step(uint32_t t, int threads, bool rec_contacts) {
     const unsigned G = (unsigned)PArr.size();
     zero_reaction(); // setzero_reaction
-    // run_timeloop_worm (timeloop.h:1017-1369): the same step except for
-    // the lines marked "worm"; it has no timed toggles, ramp or torque
-    const bool worm = TL.timeloop_worm != 0;
+    // run_timeloop_worm (timeloop.h:1017-1369) and run_timeloop_compact
+    // (:335-498): the same step except for the lines marked "ungated" (both)
+    // and "worm"; neither has timed toggles or torque, the worm loop no ramp
+    const bool ungated = TL.timeloop_variant != 0;
+    const bool worm = TL.timeloop_variant == 1;
     // worm: clamp the front (0, 3) or the back (1, 2) nodes (timeloop.h:1106-1116)
     const int t_num = ((t / 1000) % (2 + 1) == 0) ? 0 : 1;
-    if (!worm && TL.set_movable_index != -1) {
+    if (!ungated && TL.set_movable_index != -1) {
       unsigned pi = (unsigned)TL.set_movable_index, ts = (unsigned)TL.set_movable_timestep;
       if (ts == t && pi < G) PArr[pi].movable = 1;
     }
-    if (!worm && TL.set_stoppable_index != -1) {
+    if (!ungated && TL.set_stoppable_index != -1) {
       unsigned pi = (unsigned)TL.set_stoppable_index, ts = (unsigned)TL.set_stoppable_timestep;
       if (ts == t && pi < G) PArr[pi].stoppable = 1;
     }
-    if (!worm && TL.reset_partzero_y && G > 1) {
+    if (!ungated && TL.reset_partzero_y && G > 1) {
       if (TL.reset_partzero_y_timestep == t) {
         double max_bulk_y = sval(PArr[1].pos[0](1) + PArr[1].disp[1](1));
         for (unsigned i = 1; i < G; ++i)
@@ -581,7 +583,7 @@ template <unsigned D, class T> struct Oracle : OracleBase {
 #pragma omp parallel for num_threads(threads) if (threads > 1)
     for (unsigned i = 0; i < G; ++i) {
       Grain<D, T> &P = PArr[i];
-      if (P.movable || worm) { // worm: every grain (timeloop.h:1120-1124)
+      if (P.movable || ungated) { // ungated: every grain (timeloop.h:364-366, 1120-1124)
         const double h = (dt * dt * 0.5);
         for (unsigned n = 0; n < P.nnodes; ++n) {
           V inc = (dt * P.vel[n]) + (h * P.acc[n]);
@@ -611,7 +613,7 @@ template <unsigned D, class T> struct Oracle : OracleBase {
               contact_by_boundary<D, T>(P, P, CN, dt, ft, true, pairs, node_off[i], node_off[i]);
           }
         }
-        if (P.stoppable || worm) { // worm: contact not gated (timeloop.h:1160)
+        if (P.stoppable || ungated) { // ungated: contact not gated (timeloop.h:404, 1160)
           const double *C = &bbox[i * 6];
           for (unsigned j = 0; j < G; j++) {
             if (j != i) {
@@ -634,7 +636,7 @@ template <unsigned D, class T> struct Oracle : OracleBase {
         } else {
           for (unsigned n = 0; n < P.nnodes; ++n) ft[n] += P.extforce[n];
         }
-        if (!worm && TL.enable_torque) {
+        if (!ungated && TL.enable_torque) {
           V c = P.mean_CurrPos();
           unsigned taxis = P.torque_axis;
           for (unsigned nn = 0; nn < P.nnodes; nn++) {

@@ -79,7 +79,7 @@ class PgParams(ctypes.Structure):
         ("reset_partzero_y_timestep", ctypes.c_uint32),
         ("wheel_rad", ctypes.c_double),
         ("wall_speed", ctypes.c_double * PG_MAX_SIDES),
-        ("timeloop_worm", ctypes.c_int32),
+        ("timeloop_variant", ctypes.c_int32),
     ]
 
 
@@ -151,7 +151,7 @@ def params_struct(p):
     pp.wheel_rad = float(p.wheel_rad)
     for k in range(PG_MAX_SIDES):
         pp.wall_speed[k] = float(p.wall_speed[k])
-    pp.timeloop_worm = int(getattr(p, "timeloop_worm", 0))
+    pp.timeloop_variant = int(getattr(p, "timeloop_variant", 0))
     return pp
 
 

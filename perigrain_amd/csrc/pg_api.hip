@@ -432,6 +432,8 @@ int check_setup(const pg_setup *s, const pg_params *p, int dim_expected) {
   if (!chk_idx(p->set_movable_index, p->set_movable_timestep) ||
       !chk_idx(p->set_stoppable_index, p->set_stoppable_timestep))
     return fail(PG_ERR_INVALID, "set_movable_index/set_stoppable_index out of range");
+  if (p->timeloop_variant < PG_TIMELOOP_STANDARD || p->timeloop_variant > PG_TIMELOOP_COMPACT)
+    return fail(PG_ERR_INVALID, "timeloop_variant must be 0, 1 or 2");
   return PG_OK;
 }
 
@@ -466,7 +468,8 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
     ih.skin = h->skin_factor * (st.contact_radius > 0 ? st.contact_radius : 0.0);
     ih.nsides = D == 2 ? 4 : 6;
     const pg_params &pp = params[s];
-    ih.reset_step = (pp.reset_partzero_y && !pp.timeloop_worm) ? pp.reset_partzero_y_timestep : 0;
+    ih.reset_step = (pp.reset_partzero_y && pp.timeloop_variant == PG_TIMELOOP_STANDARD)
+                        ? pp.reset_partzero_y_timestep : 0;
     ih.wheel_rad = pp.wheel_rad;
   }
   // record gathers use 32-bit byte offsets of 16 bytes per node
@@ -511,8 +514,8 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         node_grain[i] = gg;
         if (!bdry_only && ih.contact) node_info[i] |= NI_CONTACT;
       }
-      if (pp.timeloop_worm) { // nodes 0, 3 (front) and 1, 2 (back) of each grain
-        if (b - a < 4) return fail(PG_ERR_INVALID, "timeloop_worm needs four nodes in every grain");
+      if (pp.timeloop_variant == PG_TIMELOOP_WORM) { // nodes 0, 3 (front), 1, 2 (back)
+        if (b - a < 4) return fail(PG_ERR_INVALID, "the worm loop needs four nodes in every grain");
         node_info[ih.node_begin + a + 0] |= NI_WORM_FRONT;
         node_info[ih.node_begin + a + 3] |= NI_WORM_FRONT;
         node_info[ih.node_begin + a + 1] |= NI_WORM_BACK;
@@ -583,9 +586,11 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
                (pp.allow_damping ? IF_DAMPING : 0) | (pp.allow_friction ? IF_FRICTION : 0) |
                (pp.self_contact ? IF_SELF_CONTACT : 0) | (st.allow_wall ? IF_WALL : 0) |
                (pp.gradient_extforce ? IF_GRAD_EXT : 0) | (pp.enable_torque ? IF_TORQUE : 0);
-    // run_timeloop_worm has no extforce ramp, torque or timed toggles
-    const bool worm = pp.timeloop_worm != 0;
-    if (worm) id.flags = (id.flags & ~(IF_GRAD_EXT | IF_TORQUE)) | IF_WORM;
+    // the worm and compact loops have no torque or timed toggles; the worm
+    // loop has no extforce ramp either
+    const bool worm = pp.timeloop_variant != PG_TIMELOOP_STANDARD;
+    if (worm) id.flags = (id.flags & ~IF_TORQUE) | IF_UNGATED;
+    if (pp.timeloop_variant == PG_TIMELOOP_WORM) id.flags = (id.flags & ~IF_GRAD_EXT) | IF_WORM;
     id.extforce_maxstep = pp.extforce_maxstep;
     id.set_movable_grain = (!worm && pp.set_movable_index >= 0 && pp.set_movable_timestep > 0)
                                ? (int32_t)(ih.grain_begin + pp.set_movable_index) : -1;
@@ -1530,10 +1535,10 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
   if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
   const InstDev &id = h->inst_dev[0];
   if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped ||
-      h->inst[0].reset_step != 0 || (id.flags & IF_WORM))
+      h->inst[0].reset_step != 0 || (id.flags & IF_UNGATED))
     return fail(PG_ERR_INVALID,
                 "the adjoint supports no torque, no timed movable/stoppable toggles, no clamped "
-                "nodes, no reset_partzero_y and not the worm loop");
+                "nodes, no reset_partzero_y and only the standard loop");
   HIPCHK(hipSetDevice(h->device));
   auto &a = h->adj;
   RC(adj_alloc(h, nsteps, checkpoint_every));

@@ -63,7 +63,10 @@ constexpr int32_t IF_SELF_CONTACT = 16;
 constexpr int32_t IF_WALL = 32;
 constexpr int32_t IF_GRAD_EXT = 64;
 constexpr int32_t IF_TORQUE = 128;
-constexpr int32_t IF_WORM = 256; // the loop of run_timeloop_worm (timeloop.h:1017-1369)
+// run_timeloop_worm and run_timeloop_compact (timeloop.h:1017-1369, 335-498):
+// every grain advances its position, contact is not gated by stoppable
+constexpr int32_t IF_UNGATED = 256;
+constexpr int32_t IF_WORM = 512; // the worm loop's phases (timeloop.h:1106-1245)
 
 // node info bits
 constexpr uint8_t NI_CLAMPED = 1;

@@ -392,7 +392,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 
       // ---- (b) contact from the sorted Verlet list
       const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
-      const bool stop = (ip.flags & IF_WORM) || grain_stoppable(gp, ip, g, t);
+      const bool stop = (ip.flags & IF_UNGATED) || grain_stoppable(gp, ip, g, t);
       double boxg[2 * D];
       load_box<D>(A.bbox[t % 3], g, boxg);
       if ((info & NI_CONTACT) && (selfc || stop)) {
@@ -536,7 +536,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 
     // ---- position update of step t+1 (timeloop.h:661-669)
     double un[D];
-    if ((ip.flags & IF_WORM) || grain_movable(gp, ip, g, t + 1)) { // worm: every grain
+    if ((ip.flags & IF_UNGATED) || grain_movable(gp, ip, g, t + 1)) { // worm, compact: every grain
 #pragma unroll
       for (int k = 0; k < D; ++k) un[k] = ui[k] + ((ip.dt * vi[k]) + (ip.h * ai[k]));
     } else {
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) void k_prologue(const StepArgs A) {
     double Xc[D], un[D], u0[D];
     pos_X<D>(A, ip, g, i, Xc);
     rec_u<D>(A.rec[0], A.ns, i, u0);
-    const bool mov = (ip.flags & IF_WORM) || grain_movable(gp, ip, g, 1);
+    const bool mov = (ip.flags & IF_UNGATED) || grain_movable(gp, ip, g, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double v = A.vel[0][(size_t)k * ns + i];
@@ -1040,7 +1040,7 @@ __global__ void k_pairs(const StepArgs A, const uint32_t t, uint32_t *cnt, const
   const bool mov = grain_movable(gp, ip, g, t);
   const bool brk = (ip.flags & IF_FRACTURE) && (gp.flags & GF_BREAKABLE);
   const bool selfc = brk && (ip.flags & IF_SELF_CONTACT);
-  const bool stop = (ip.flags & IF_WORM) || grain_stoppable(gp, ip, g, t);
+  const bool stop = (ip.flags & IF_UNGATED) || grain_stoppable(gp, ip, g, t);
   if (mov && (info & NI_CONTACT) && (selfc || stop)) {
     const double *U = A.rec[t & 1];
     double Xc[D], xc[D], box[2 * D], ui[D];

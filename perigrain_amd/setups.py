@@ -96,7 +96,7 @@ class Params:
     reset_partzero_y_timestep: int = 0
     wheel_rad: float = 0.0
     wall_speed: np.ndarray = field(default_factory=lambda: np.zeros(6))
-    timeloop_worm: int = 0
+    timeloop_variant: int = 0  # 0 run_timeloop, 1 run_timeloop_worm, 2 run_timeloop_compact
 
 
 # ----------------------------------------------------------------- materials

@@ -159,10 +159,28 @@ def test_worm_loop_bit_exact(oracle_threads):
     # held and extforce on before it, back nodes held and extforce off after)
     s, p = setups.two_disk_collision(seed=3, gap=5e-5)
     s.extforce[:] = [0.0, -9.81 * 1200.0]
-    p.timeloop_worm = 1
+    p.timeloop_variant = 1
     g, o = _both(s, p, oracle_threads)
     done = 0
     for t in (10, 999, 1000, 1001, 1100):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    assert len(o.contacts()) > 0, "the disks should be in contact"
+
+
+def test_compact_loop_bit_exact(oracle_threads):
+    # run_timeloop_compact: disk 1 is not stoppable but still makes contact,
+    # the extforce ramp acts, torque does not exist in this loop
+    s, p = setups.two_disk_collision(seed=3, gap=5e-5)
+    s.extforce[:] = [0.0, -9.81 * 1200.0]
+    s.stoppable[1] = 0
+    s.torque_axis = np.full(s.n_grains, 2, dtype=np.uint32)
+    s.torque_val = np.full(s.n_grains, 1e6)
+    p.timeloop_variant, p.gradient_extforce, p.extforce_maxstep, p.enable_torque = 2, 1, 200, 1
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1, 100, 400):
         _advance(g, o, t - done)
         done = t
         _assert_exact(g, o, f"t={t}")

@@ -460,7 +460,7 @@ def test_worm_loop_phases():
     nodes 0-3 never move in y; extforce acts only in the front-held phases;
     every grain advances its position."""
     pos = np.array([[0.0, 0.0], [1e-3, 0.0], [2e-3, 0.0], [3e-3, 0.0], [4e-3, 0.0]])
-    s, p = build([grain(pos, rho=1000.0)], 2, vels=[np.array([1.0, 1.0])], timeloop_worm=1)
+    s, p = build([grain(pos, rho=1000.0)], 2, vels=[np.array([1.0, 1.0])], timeloop_variant=1)
     s.extforce[:] = [2.0, 4.0]
     o = oracle.Oracle(s, p)
     o.step(10)  # t = 10: front held, extforce on
@@ -476,11 +476,37 @@ def test_worm_loop_phases():
     assert np.all(a[[0, 3, 4]] == 0), "no extforce while the back nodes are held"
     assert v[4, 1] > 1.0, "node 4 kept the velocity gained in the first phase"
     # a non-movable grain still advances its position in the worm loop
-    s2, p2 = build([grain(pos)], 2, vels=[np.array([1.0, 0.0])], timeloop_worm=1)
+    s2, p2 = build([grain(pos)], 2, vels=[np.array([1.0, 0.0])], timeloop_variant=1)
     s2.movable[:] = 0
     o2 = oracle.Oracle(s2, p2)
     o2.step(5)
     assert o2.read("CurrPos")[4, 0] > pos[4, 0]
+
+
+def test_compact_loop_is_ungated():
+    """run_timeloop_compact (timeloop.h:335-498): contact is computed whatever
+    stoppable says, every grain advances its position, and the loop has no
+    torque."""
+    r, Rc, K, Vj = 1.5e-5, 2e-5, 5e21, 3e-12
+    s, p = _pair(2, r, Rc=Rc, Kn=K, Vj=Vj, allow_damping=0, allow_friction=0)
+    s.stoppable[:] = 0
+    F, _ = first_force(s, p)
+    assert np.all(F == 0), "the standard loop skips contact of non-stoppable grains"
+    p.timeloop_variant = 2
+    F, _ = first_force(s, p)
+    assert F[0, 0] == pytest.approx(-K * (Rc - r) * Vj, rel=1e-12)
+    pos = np.array([[0.0, 0.0], [1e-3, 0.0]])
+    s2, p2 = build([grain(pos)], 2, vels=[np.array([1.0, 0.0])], timeloop_variant=2)
+    s2.movable[:] = 0
+    o2 = oracle.Oracle(s2, p2)
+    o2.step(3)
+    assert o2.read("CurrPos")[1, 0] > pos[1, 0], "non-movable grains still advance"
+    for variant, torque_force in ((0, True), (2, False)):
+        s3, p3 = build([grain(pos)], 2, timeloop_variant=variant, enable_torque=1)
+        s3.torque_axis = np.array([2], dtype=np.uint32)
+        s3.torque_val = np.array([5.0])
+        F, _ = first_force(s3, p3)
+        assert np.any(F != 0) == torque_force
 
 
 def test_torque_is_tangential():
