@@ -89,231 +89,250 @@ __device__ __forceinline__ bool pair_gate(const AdjArgs &B, uint32_t t, const Gr
 
 } // namespace
 
+// One node per ADJ_LANES consecutive lanes: lane r of node i takes slot r of
+// each column group (BGROUP == ADJ_LANES slots), contact entries r, r + L,
+// ... of i's list and wall sides r, r + L, ...; the lanes' partial
+// cotangents are summed with shuffles and lane 0 writes the node's result.
+// The adjoint has no reference order to keep, so the sums run in any fixed
+// order.
 template <int D>
 __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_t t) {
   constexpr int N = 2 * D + 2; // dual directions: x_i (D), v_i (D), K, mu
+  constexpr uint32_t L = ADJ_LANES;
+  static_assert(L == BGROUP, "one column-group slot per lane");
   using DN = Dual<N>;
   const StepArgs &A = B.A;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = gid / L, r = gid % L;
   const uint32_t ns = A.ns;
   double gc = 0.0, gk = 0.0, gm = 0.0;
+  double au[D], av[D], Pu[D], lv[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    au[k] = 0.0;
+    av[k] = 0.0;
+    Pu[k] = 0.0;
+    lv[k] = 0.0;
+  }
+  bool act = false; // node i exists and its grain moves (the same in all L lanes)
+  uint32_t g = 0;
   if (i < A.n_nodes) {
-    const uint32_t g = A.node_grain[i];
+    g = A.node_grain[i];
+    act = (A.grains[g].flags & GF_MOVABLE) != 0;
+  }
+  if (act) {
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
-    if (gp.flags & GF_MOVABLE) {
-      double Pu[D], Pv[D], Pa[D], lv[D], lF[D];
-      load_cols<D>(B.Pu, ns, i, Pu);
-      load_cols<D>(B.Pv, ns, i, Pv);
-      load_cols<D>(B.Pa, ns, i, Pa);
-      force_cotangent<D>(ip, gp.rho, Pu, Pv, Pa, lv, lF);
-      double Xi[D], ui[D], xi[D], vi[D], au[D], av[D];
-      rec_X<D>(A, i, Xi);
-      rec_u<D>(B.su, A.ns, i, ui);
-      load_cols<D>(B.sv, ns, i, vi);
+    double Pv[D], Pa[D], lF[D];
+    load_cols<D>(B.Pu, ns, i, Pu);
+    load_cols<D>(B.Pv, ns, i, Pv);
+    load_cols<D>(B.Pa, ns, i, Pa);
+    force_cotangent<D>(ip, gp.rho, Pu, Pv, Pa, lv, lF);
+    double Xi[D], ui[D], xi[D], vi[D];
+    rec_X<D>(A, i, Xi);
+    rec_u<D>(B.su, A.ns, i, ui);
+    load_cols<D>(B.sv, ns, i, vi);
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        xi[k] = Xi[k] + ui[k];
-        au[k] = 0.0;
-        av[k] = 0.0;
-      }
-      const double Vi = A.V[i];
-      const double c = gp.cnot;
+    for (int k = 0; k < D; ++k) xi[k] = Xi[k] + ui[k];
+    const double Vi = A.V[i];
+    const double c = gp.cnot;
 
-      // ---- bonds: F_i = sum_j c V_j s e,  dF_i/du_j = c V_j M,  M symmetric.
-      // Slots are taken four at a time (one 16-byte load of column words and
-      // one of break steps); the four partners' data are loaded together.
-      // Padding slots and bonds removed after an earlier step read the ghost
-      // (finite values) and their terms are selected away.
-      {
-        const uint32_t sl = i >> 6, lane = i & 63;
-        const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-        const uint32_t *bc = A.bcol + base + BGROUP * lane;
-        const uint32_t *bk = A.break_step + base + BGROUP * lane;
-        for (uint32_t k0 = 0; k0 < width; k0 += BGROUP) {
-          const uint4 cw4 = *reinterpret_cast<const uint4 *>(bc + SLICE * k0);
-          const uint4 bk4 = *reinterpret_cast<const uint4 *>(bk + SLICE * k0);
-          const uint32_t cws[4] = {cw4.x, cw4.y, cw4.z, cw4.w};
-          const uint32_t bks[4] = {bk4.x, bk4.y, bk4.z, bk4.w};
-          double Xj[4][D], uj[4][D], lFj[4][D], Vj[4];
-          bool live[4];
+    // ---- bonds: F_i = sum_j c V_j s e,  dF_i/du_j = c V_j M,  M symmetric.
+    // Padding slots and bonds removed after an earlier step read the ghost
+    // (finite values) and their terms are selected away.
+    {
+      const uint32_t sl = i >> 6, lane = i & 63;
+      const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+      const uint32_t *bc = A.bcol + base + BGROUP * lane + r;
+      const uint32_t *bk = A.break_step + base + BGROUP * lane + r;
+      for (uint32_t k0 = 0; k0 < width; k0 += BGROUP) {
+        const uint32_t cw = bc[SLICE * k0], bks = bk[SLICE * k0];
+        const bool live = cw != BCOL_PAD && bks >= t; // a bond broken at t is still used at t
+        const uint32_t j = live ? (cw & BCOL_INDEX) : A.ghost;
+        double Xj[D], uj[D], lFj[D];
+        rec_X<D>(A, j, Xj);
+        rec_u<D>(B.su, A.ns, j, uj);
+        load_cols<D>(B.lf_in, ns, j, lFj);
+        const double Vj = A.V[j];
+        double xb[D], eta[D], e[D], q[D];
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            live[m] = cws[m] != BCOL_PAD && bks[m] >= t; // a bond broken at t is still used at t
-            const uint32_t j = live[m] ? (cws[m] & BCOL_INDEX) : A.ghost;
-            rec_X<D>(A, j, Xj[m]);
-            rec_u<D>(B.su, A.ns, j, uj[m]);
-            load_cols<D>(B.lf_in, ns, j, lFj[m]);
-            Vj[m] = A.V[j];
+        for (int m = 0; m < D; ++m) {
+          xb[m] = Xj[m] - Xi[m];
+          eta[m] = (uj[m] - ui[m]) + xb[m];
+        }
+        const double xn = sqrt(red<D>(xb, xb));
+        const double n = sqrt(red<D>(eta, eta));
+        const double s = (n - xn) / xn;
+        double eq = 0.0, el = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+          e[m] = eta[m] / n;
+          q[m] = c * Vi * lFj[m] - c * Vj * lF[m];
+          eq += e[m] * q[m];
+          el += e[m] * lF[m];
+        }
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+          const double du = e[m] * eq / xn + s * (q[m] - e[m] * eq) / n;
+          au[m] += live ? du : 0.0;
+        }
+        const double dc = c * Vj * s * el;
+        gc += live ? dc : 0.0;
+      }
+    }
+
+    const uint8_t info = A.node_info[i];
+    const bool stop = (gp.flags & GF_STOPPABLE) != 0;
+
+    // ---- node-node contact over the segment's candidate list (i's own
+    // terms are gated by i's grain, the partners' terms by theirs)
+    if (info & NI_CONTACT) {
+      const uint32_t c0 = A.cl_ptr[i], c1 = A.cl_ptr[i + 1];
+      for (uint32_t cc = c0 + r; cc < c1; cc += L) {
+        // a same-grain pair still bonded at step t has no contact in either
+        // direction (its two slots break in the same step): skip it before
+        // loading the partner
+        const int32_t bs = A.cl_slot[cc];
+        if (bs >= 0 && A.break_step[bs] > t) continue;
+        const uint32_t j = A.cl_j[cc];
+        const uint32_t gj = A.node_grain[j];
+        const GrainDev gpj = A.grains[gj];
+        double Xj[D], uj[D], xj[D], vj[D];
+        rec_X<D>(A, j, Xj);
+        rec_u<D>(B.su, A.ns, j, uj);
+        load_cols<D>(B.sv, ns, j, vj);
+#pragma unroll
+        for (int k = 0; k < D; ++k) xj[k] = Xj[k] + uj[k];
+        // i's force from j: seeds on x_i, v_i, K, mu
+        double R, K;
+        bool self;
+        if (pair_gate<D>(B, t, gp, g, ip, gj, Xi, Xj, bs, R, K, self)) {
+          DN d[D], w[D], F[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            d[k] = DN(Xj[k] + uj[k]) - DN::seed(xi[k], k);
+            w[k] = DN::seed(vi[k], D + k) - DN(vj[k]);
+            F[k] = DN(0.0);
           }
+          const DN rr = sqrt(red<D, N>(d, d));
+          if (rr.v <= R) {
+            pair_force<D, DN>(d, rr, R, DN::seed(K, 2 * D), A.V[j], w, gp.rho,
+                              DN::seed(ip.mu, 2 * D + 1), ip, F);
+            double gK = 0.0;
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            double xb[D], eta[D], e[D], q[D];
+            for (int k = 0; k < D; ++k) {
 #pragma unroll
-            for (int r = 0; r < D; ++r) {
-              xb[r] = Xj[m][r] - Xi[r];
-              eta[r] = (uj[m][r] - ui[r]) + xb[r];
+              for (int m = 0; m < D; ++m) {
+                au[m] += F[k].d[m] * lF[k];
+                av[m] += F[k].d[D + m] * lF[k];
+              }
+              gK += F[k].d[2 * D] * lF[k];
+              gm += F[k].d[2 * D + 1] * lF[k];
             }
-            const double xn = sqrt(red<D>(xb, xb));
-            const double n = sqrt(red<D>(eta, eta));
-            const double s = (n - xn) / xn;
-            double eq = 0.0, el = 0.0;
-#pragma unroll
-            for (int r = 0; r < D; ++r) {
-              e[r] = eta[r] / n;
-              q[r] = c * Vi * lFj[m][r] - c * Vj[m] * lF[r];
-              eq += e[r] * q[r];
-              el += e[r] * lF[r];
-            }
-#pragma unroll
-            for (int r = 0; r < D; ++r) {
-              const double du = e[r] * eq / xn + s * (q[r] - e[r] * eq) / n;
-              au[r] += live[m] ? du : 0.0;
-            }
-            const double dc = c * Vj[m] * s * el;
-            gc += live[m] ? dc : 0.0;
+            if (self) gc += gK * K; // K = c/rd: d/d(c scale) = K
+            else gk += gK;
           }
         }
-      }
-
-      const uint8_t info = A.node_info[i];
-      const bool stop = (gp.flags & GF_STOPPABLE) != 0;
-
-      // ---- node-node contact over the segment's candidate list (i's own
-      // terms are gated by i's grain, the partners' terms by theirs)
-      if (info & NI_CONTACT) {
-        const uint32_t c0 = A.cl_ptr[i], c1 = A.cl_ptr[i + 1];
-        for (uint32_t cc = c0; cc < c1; ++cc) {
-          // a same-grain pair still bonded at step t has no contact in either
-          // direction (its two slots break in the same step): skip it before
-          // loading the partner
-          const int32_t bs = A.cl_slot[cc];
-          if (bs >= 0 && A.break_step[bs] > t) continue;
-          const uint32_t j = A.cl_j[cc];
-          const uint32_t gj = A.node_grain[j];
-          const GrainDev gpj = A.grains[gj];
-          double Xj[D], uj[D], xj[D], vj[D];
-          rec_X<D>(A, j, Xj);
-          rec_u<D>(B.su, A.ns, j, uj);
-          load_cols<D>(B.sv, ns, j, vj);
-#pragma unroll
-          for (int k = 0; k < D; ++k) xj[k] = Xj[k] + uj[k];
-          // i's force from j: seeds on x_i, v_i, K, mu
-          double R, K;
-          bool self;
-          if (pair_gate<D>(B, t, gp, g, ip, gj, Xi, Xj, bs, R, K, self)) {
+        // j's force from i (computed by j when j's grain moves): seeds on
+        // x_i, v_i only; j accounts for its own parameter derivatives
+        if ((gpj.flags & GF_MOVABLE)) {
+          double Rj, Kj;
+          bool selfj;
+          if (pair_gate<D>(B, t, gpj, gj, ip, g, Xj, Xi, bs, Rj, Kj, selfj)) {
             DN d[D], w[D], F[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-              d[k] = DN(Xj[k] + uj[k]) - DN::seed(xi[k], k);
-              w[k] = DN::seed(vi[k], D + k) - DN(vj[k]);
+              d[k] = DN::seed(xi[k], k) - DN(xj[k]);
+              w[k] = DN(vj[k]) - DN::seed(vi[k], D + k);
               F[k] = DN(0.0);
             }
-            const DN r = sqrt(red<D, N>(d, d));
-            if (r.v <= R) {
-              pair_force<D, DN>(d, r, R, DN::seed(K, 2 * D), A.V[j], w, gp.rho,
-                                DN::seed(ip.mu, 2 * D + 1), ip, F);
-              double gK = 0.0;
+            const DN rr = sqrt(red<D, N>(d, d));
+            if (rr.v <= Rj) {
+              double lFj[D];
+              load_cols<D>(B.lf_in, ns, j, lFj);
+              pair_force<D, DN>(d, rr, Rj, DN(Kj), Vi, w, gpj.rho, DN(ip.mu), ip, F);
 #pragma unroll
-              for (int k = 0; k < D; ++k) {
+              for (int k = 0; k < D; ++k)
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
-                  au[m] += F[k].d[m] * lF[k];
-                  av[m] += F[k].d[D + m] * lF[k];
+                  au[m] += F[k].d[m] * lFj[k];
+                  av[m] += F[k].d[D + m] * lFj[k];
                 }
-                gK += F[k].d[2 * D] * lF[k];
-                gm += F[k].d[2 * D + 1] * lF[k];
-              }
-              if (self) gc += gK * K; // K = c/rd: d/d(c scale) = K
-              else gk += gK;
-            }
-          }
-          // j's force from i (computed by j when j's grain moves): seeds on
-          // x_i, v_i only; j accounts for its own parameter derivatives
-          if ((gpj.flags & GF_MOVABLE)) {
-            double Rj, Kj;
-            bool selfj;
-            if (pair_gate<D>(B, t, gpj, gj, ip, g, Xj, Xi, bs, Rj, Kj, selfj)) {
-              DN d[D], w[D], F[D];
-#pragma unroll
-              for (int k = 0; k < D; ++k) {
-                d[k] = DN::seed(xi[k], k) - DN(xj[k]);
-                w[k] = DN(vj[k]) - DN::seed(vi[k], D + k);
-                F[k] = DN(0.0);
-              }
-              const DN r = sqrt(red<D, N>(d, d));
-              if (r.v <= Rj) {
-                double lFj[D];
-                load_cols<D>(B.lf_in, ns, j, lFj);
-                pair_force<D, DN>(d, r, Rj, DN(Kj), Vi, w, gpj.rho, DN(ip.mu), ip, F);
-#pragma unroll
-                for (int k = 0; k < D; ++k)
-#pragma unroll
-                  for (int m = 0; m < D; ++m) {
-                    au[m] += F[k].d[m] * lFj[k];
-                    av[m] += F[k].d[D + m] * lFj[k];
-                  }
-              }
             }
           }
         }
       }
+    }
 
-      // ---- walls: cotangent of side s's force on i is lF - V_i lR_s
-      if (stop && (ip.flags & IF_WALL) && (info & NI_CONTACT)) {
-        const double *wp = B.swall + (size_t)gp.inst * 6;
-        double boxg[2 * D];
-        load_box<D>(B.sbox, g, boxg);
-        if (!within_interior<D>(boxg, wp, ip.Rc)) {
-          const double *lR = B.lR + (size_t)gp.inst * 6 * D;
-          constexpr int NS = (D == 2) ? 4 : 6;
-          const int sides2[4] = {3, 0, 1, 2}, dirs2[4] = {1, 0, 0, 1};
-          const int sides3[6] = {0, 3, 1, 4, 2, 5}, dirs3[6] = {0, 0, 1, 1, 2, 2};
+    // ---- walls: cotangent of side s's force on i is lF - V_i lR_s
+    if (stop && (ip.flags & IF_WALL) && (info & NI_CONTACT)) {
+      const double *wp = B.swall + (size_t)gp.inst * 6;
+      double boxg[2 * D];
+      load_box<D>(B.sbox, g, boxg);
+      if (!within_interior<D>(boxg, wp, ip.Rc)) {
+        const double *lR = B.lR + (size_t)gp.inst * 6 * D;
+        constexpr int NS = (D == 2) ? 4 : 6;
+        const int sides2[4] = {3, 0, 1, 2}, dirs2[4] = {1, 0, 0, 1};
+        const int sides3[6] = {0, 3, 1, 4, 2, 5}, dirs3[6] = {0, 0, 1, 1, 2, 2};
 #pragma unroll
-          for (int q = 0; q < NS; ++q) {
-            const int side = (D == 2) ? sides2[q] : sides3[q];
-            const int dir = (D == 2) ? dirs2[q] : dirs3[q];
-            DN v[D], F[D];
+        for (int qq = 0; qq < NS; ++qq) {
+          if ((uint32_t)qq % L != r) continue;
+          const int side = (D == 2) ? sides2[qq] : sides3[qq];
+          const int dir = (D == 2) ? dirs2[qq] : dirs3[qq];
+          DN v[D], F[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            v[k] = DN::seed(vi[k], D + k);
+            F[k] = DN(0.0);
+          }
+          const DN dist = DN::seed(xi[dir], dir) - DN(wp[side]);
+          if (wall_side<D, DN>(dist, dir, ip, DN::seed(ip.Kn, 2 * D),
+                               DN::seed(ip.mu, 2 * D + 1), gp.rho, Vi, ip.speed[side], v, F,
+                               nullptr)) {
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-              v[k] = DN::seed(vi[k], D + k);
-              F[k] = DN(0.0);
-            }
-            const DN dist = DN::seed(xi[dir], dir) - DN(wp[side]);
-            if (wall_side<D, DN>(dist, dir, ip, DN::seed(ip.Kn, 2 * D),
-                                 DN::seed(ip.mu, 2 * D + 1), gp.rho, Vi, ip.speed[side], v, F,
-                                 nullptr)) {
+              const double cot = lF[k] - Vi * lR[side * D + k];
 #pragma unroll
-              for (int k = 0; k < D; ++k) {
-                const double cot = lF[k] - Vi * lR[side * D + k];
-#pragma unroll
-                for (int m = 0; m < D; ++m) {
-                  au[m] += F[k].d[m] * cot;
-                  av[m] += F[k].d[D + m] * cot;
-                }
-                gk += F[k].d[2 * D] * cot;
-                gm += F[k].d[2 * D + 1] * cot;
+              for (int m = 0; m < D; ++m) {
+                au[m] += F[k].d[m] * cot;
+                av[m] += F[k].d[D + m] * cot;
               }
+              gk += F[k].d[2 * D] * cot;
+              gm += F[k].d[2 * D + 1] * cot;
             }
           }
         }
       }
-
-      // ---- cotangents of step t-1
-      double nPu[D], nPv[D], nPa[D], nlv[D], nlF[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        nPu[k] = Pu[k] + au[k];
-        nPv[k] = lv[k] + av[k];
-        nPa[k] = ip.half_dt * lv[k];
-        B.Pu[(size_t)k * ns + i] = nPu[k];
-        B.Pv[(size_t)k * ns + i] = nPv[k];
-        B.Pa[(size_t)k * ns + i] = nPa[k];
-      }
-      force_cotangent<D>(ip, gp.rho, nPu, nPv, nPa, nlv, nlF);
-#pragma unroll
-      for (int k = 0; k < D; ++k) B.lf_out[(size_t)k * ns + i] = nlF[k];
     }
+  }
+
+  // the node's L lanes are consecutive and share `act`: sum their partial
+  // cotangents (every lane of the wavefront takes part in the shuffles)
+#pragma unroll
+  for (uint32_t o = 1; o < L; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      au[k] += __shfl_xor(au[k], (int)o);
+      av[k] += __shfl_xor(av[k], (int)o);
+    }
+
+  // ---- cotangents of step t-1
+  if (act && r == 0) {
+    const GrainDev gp = A.grains[g];
+    const InstDev &ip = A.insts[gp.inst];
+    double nPu[D], nPv[D], nPa[D], nlv[D], nlF[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      nPu[k] = Pu[k] + au[k];
+      nPv[k] = lv[k] + av[k];
+      nPa[k] = ip.half_dt * lv[k];
+      B.Pu[(size_t)k * ns + i] = nPu[k];
+      B.Pv[(size_t)k * ns + i] = nPv[k];
+      B.Pa[(size_t)k * ns + i] = nPa[k];
+    }
+    force_cotangent<D>(ip, gp.rho, nPu, nPv, nPa, nlv, nlF);
+#pragma unroll
+    for (int k = 0; k < D; ++k) B.lf_out[(size_t)k * ns + i] = nlF[k];
   }
 
   // deterministic block sums of the parameter partials

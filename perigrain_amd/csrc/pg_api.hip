@@ -1160,7 +1160,7 @@ int adj_alloc(pg_handle *h, uint32_t T, uint32_t K) {
     RC(dalloc(h, a.Pa, vec_len(h)));
     RC(dalloc(h, a.lf[0], vec_len(h)));
     RC(dalloc(h, a.lf[1], vec_len(h)));
-    RC(dalloc(h, a.gpart, (size_t)((h->N + 255) / 256 + 1) * 3));
+    RC(dalloc(h, a.gpart, (((size_t)h->N * ADJ_LANES + 255) / 256 + 1) * 3));
     RC(dalloc(h, a.dmax, 1));
     RC(dalloc(h, a.fin_rec, rec_len(h)));
     RC(dalloc(h, a.fin_force, vec_len(h)));
@@ -1590,7 +1590,7 @@ int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad) 
   HIPCHK(hipMemsetAsync(a.Pa, 0, vec_len(h) * sizeof(double), h->stream));
   HIPCHK(hipMemsetAsync(a.lf[0], 0, vec_len(h) * sizeof(double), h->stream));
   HIPCHK(hipMemsetAsync(a.lf[1], 0, vec_len(h) * sizeof(double), h->stream));
-  const uint32_t blocks = (h->N + 255) / 256;
+  const uint32_t blocks = (uint32_t)(((size_t)h->N * ADJ_LANES + 255) / 256);
   HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
   InstHost &ih = h->inst[0];
   const double skin0 = ih.skin;
