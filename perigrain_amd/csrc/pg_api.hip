@@ -1177,26 +1177,58 @@ int adj_alloc(pg_handle *h, uint32_t T, uint32_t K) {
 int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
   auto &a = h->adj;
   const uint32_t end = c + n;
+  // What step s of the segment reads (u^s with X, v^{s-1}, walls, boxes)
+  // lives in slot s-c-1 of the segment buffers, and each step writes its
+  // results straight into the next slot, so the snapshots cost no copies.
+  // Past the segment the handle's own buffers take the writes (they are
+  // scratch until the next checkpoint restore).  The slots' X parts were
+  // filled once by pg_adjoint_backward.
+  auto seg = [&](uint32_t s) { return s > c && s - c - 1 < n; };
+  auto Rs = [&](uint32_t s) {
+    return seg(s) ? a.su + (size_t)(s - c - 1) * rec_len(h) : h->rec[s & 1];
+  };
+  auto Vs = [&](uint32_t s) {
+    return seg(s) ? a.sv + (size_t)(s - c - 1) * vec_len(h) : h->vel[(s + 1) & 1];
+  };
+  auto Ws = [&](uint32_t s) {
+    return seg(s) ? a.swall + (size_t)(s - c - 1) * h->I * 6 : h->wallpos[s & 1];
+  };
+  auto Bs = [&](uint32_t s) {
+    return seg(s) ? a.sbox + (size_t)(s - c - 1) * box_len(h) : h->bbox[s % 3];
+  };
+  // slot 0: the restored state after step c; the boxes step c+1 collects
+  // start from the cleared buffer ck_restore left for them
+  {
+    CopySet cs;
+    auto w8 = [](const void *p) { return (unsigned long long *)const_cast<void *>(p); };
+    cs.dst[0] = w8(Rs(c + 1));
+    cs.src[0] = w8(h->rec[(c + 1) & 1]);
+    cs.n[0] = rec_len(h);
+    cs.dst[1] = w8(Vs(c + 1));
+    cs.src[1] = w8(h->vel[c & 1]);
+    cs.n[1] = vec_len(h);
+    cs.dst[2] = w8(Ws(c + 1));
+    cs.src[2] = w8(h->wallpos[(c + 1) & 1]);
+    cs.n[2] = (size_t)h->I * 6;
+    cs.dst[3] = w8(Bs(c + 1));
+    cs.src[3] = w8(h->bbox[(c + 1) % 3]);
+    cs.n[3] = box_len(h);
+    HIPCHK(launch_copy4(cs, h->stream));
+    if (Bs(c + 2) != h->bbox[(c + 2) % 3]) RC(d2d(h, Bs(c + 2), h->bbox[(c + 2) % 3], box_len(h)));
+  }
   while (h->t < end) {
     for (uint32_t t = h->t + 1; t <= end; ++t) {
-      const size_t q = t - c - 1;
-      // snapshot of what step t reads: u^t, v^{t-1}, walls and boxes
-      CopySet cs;
-      auto w8 = [](const void *p) { return (unsigned long long *)const_cast<void *>(p); };
-      cs.dst[0] = w8(a.su + q * rec_len(h));
-      cs.src[0] = w8(h->rec[t & 1]);
-      cs.n[0] = rec_len(h);
-      cs.dst[1] = w8(a.sv + q * vec_len(h));
-      cs.src[1] = w8(h->vel[(t + 1) & 1]);
-      cs.n[1] = vec_len(h);
-      cs.dst[2] = w8(a.swall + q * h->I * 6);
-      cs.src[2] = w8(h->wallpos[t & 1]);
-      cs.n[2] = (size_t)h->I * 6;
-      cs.dst[3] = w8(a.sbox + q * box_len(h));
-      cs.src[3] = w8(h->bbox[t % 3]);
-      cs.n[3] = box_len(h);
-      HIPCHK(launch_copy4(cs, h->stream));
-      HIPCHK(launch_step(h->dim, false, h->A, t, h->block, h->stream));
+      StepArgs As = h->A;
+      As.rec[t & 1] = Rs(t);
+      As.rec[(t + 1) & 1] = Rs(t + 1);
+      As.vel[(t + 1) & 1] = Vs(t);
+      As.vel[t & 1] = Vs(t + 1);
+      As.wallpos[t & 1] = Ws(t);
+      As.wallpos[(t + 1) & 1] = Ws(t + 1);
+      As.bbox[t % 3] = Bs(t);
+      As.bbox[(t + 1) % 3] = Bs(t + 1);
+      As.bbox[(t + 2) % 3] = Bs(t + 2);
+      HIPCHK(launch_step(h->dim, false, As, t, h->block, h->stream));
     }
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           h->stream));
@@ -1205,7 +1237,7 @@ int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
     if (v <= end) {
       if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid in the recompute");
       h->t = v - 1;
-      RC(build_list(h, v));
+      RC(build_list(h, v, Rs(v)));
       continue;
     }
     h->t = end;
@@ -1592,6 +1624,8 @@ int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad) 
   HIPCHK(hipMemsetAsync(a.lf[1], 0, vec_len(h) * sizeof(double), h->stream));
   const uint32_t blocks = (uint32_t)(((size_t)h->N * ADJ_LANES + 255) / 256);
   HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
+  // X parts of the segment's record slots (the recompute writes only u)
+  for (uint32_t q = 0; q < std::min(a.K, a.T); ++q) RC(d2d(h, a.su + (size_t)q * rec_len(h), h->rec[0], rec_len(h)));
   InstHost &ih = h->inst[0];
   const double skin0 = ih.skin;
   int rc = PG_OK;
