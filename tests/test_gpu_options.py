@@ -1,0 +1,119 @@
+"""GPU parity of the configuration switches of config/main.conf and of the
+per-grain and per-node inputs of all.h5, each against the oracle, bit for
+bit (every field, bond status, contact pairs, walls): contact over the
+nonlocal boundary only, damping and friction off, self contact off,
+non-breakable grains, the timed movable/stoppable toggles, torque, clamped
+nodes, non-uniform nodal volumes and the gradient extforce ramp, in 2D and
+3D."""
+import numpy as np
+import pytest
+
+from perigrain_amd import setups
+from test_gpu_parity import _advance, _assert_exact, _both
+
+pytestmark = pytest.mark.gpu
+
+
+def _collision():
+    return setups.two_disk_collision(seed=3, gap=5e-5, speed=20.0)
+
+
+def _fracture():
+    s, p = setups.two_disk_collision(seed=4, gap=1e-5, speed=50.0)
+    nodes = s.grain_nodes(0)
+    s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
+    return s, p
+
+
+def _compression_3d():
+    return setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-5.0)
+
+
+def _nl_bdry_only(s, p):
+    p.nl_bdry_only = 1
+
+
+def _no_damping_no_friction(s, p):
+    p.allow_damping = 0
+    p.allow_friction = 0
+
+
+def _no_friction(s, p):
+    p.allow_friction = 0
+
+
+def _no_self_contact(s, p):
+    p.self_contact = 0
+
+
+def _grain0_not_breakable(s, p):
+    s.breakable[0] = 0
+
+
+def _timed_toggles(s, p):
+    s.movable[1] = 0
+    s.stoppable[1] = 0
+    p.set_movable_index, p.set_movable_timestep = 1, 60
+    p.set_stoppable_index, p.set_stoppable_timestep = 1, 120
+
+
+def _torque(s, p):
+    p.enable_torque = 1
+    s.torque_axis = np.full(s.n_grains, 2, dtype=np.uint32)
+    s.torque_val = np.array([2e6, -1e6] + [5e5] * (s.n_grains - 2))
+
+
+def _clamped(s, p):
+    # five nodes of the last grain are held at zero displacement
+    s.clamped_offset = np.zeros(s.n_grains + 1, dtype=np.int64)
+    s.clamped_offset[-1] = 5
+    s.clamped_nodes = np.array([0, 3, 7, 11, 20], dtype=np.uint32)
+
+
+def _nonuniform_volume(s, p):
+    rng = np.random.default_rng(9)
+    s.vol = s.vol * (1.0 + 0.2 * rng.uniform(-1.0, 1.0, size=s.vol.shape))
+
+
+def _extforce_ramp(s, p):
+    s.extforce[:] = 0.0
+    s.extforce[:, -1] = -9.81e4 * s.rho[0]
+    p.gradient_extforce, p.extforce_maxstep = 1, 150
+
+
+CASES = {
+    "2d-nl_bdry_only": (_collision, [_nl_bdry_only]),
+    "2d-no-damping-no-friction": (_collision, [_no_damping_no_friction]),
+    "2d-no-friction": (_collision, [_no_friction]),
+    "2d-fracture-no-self-contact": (_fracture, [_no_self_contact]),
+    "2d-fracture-grain0-not-breakable": (_fracture, [_grain0_not_breakable]),
+    "2d-timed-toggles": (_collision, [_timed_toggles]),
+    "2d-torque": (_collision, [_torque]),
+    "2d-clamped": (_collision, [_clamped]),
+    "2d-nonuniform-volume-fracture": (_fracture, [_nonuniform_volume]),
+    "2d-extforce-ramp": (_collision, [_extforce_ramp]),
+    "3d-nl_bdry_only-nonuniform-volume": (_compression_3d, [_nl_bdry_only, _nonuniform_volume]),
+    "3d-torque-clamped": (_compression_3d, [_torque, _clamped]),
+    "3d-no-damping-no-friction": (_compression_3d, [_no_damping_no_friction]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_option_bit_exact(name, oracle_threads):
+    make, mods = CASES[name]
+    s, p = make()
+    for m in mods:
+        m(s, p)
+    print(f"case {name}: {s.n_nodes} nodes", flush=True)
+    g, o = _both(s, p, oracle_threads)
+    try:
+        done, seen = 0, 0
+        for t in (1, 61, 121, 250):
+            _advance(g, o, t - done)
+            done = t
+            _assert_exact(g, o, f"{name} t={t}")
+            seen = max(seen, len(o.contacts()))
+        assert seen > 0, f"{name}: the grains should come into contact"
+    finally:
+        g.close()
+        o.close()
