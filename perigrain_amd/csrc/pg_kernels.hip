@@ -90,21 +90,26 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
 #if PG_FAST_DIVSQRT
   // |xi| = sqrt(xin2), |eta| = sqrt(n2), s = (n - |xi|)/|xi|, e = eta/n,
   // through the shortened sequences (one reciprocal for the three e_q).
-  // Exact when xin2, n2 lie in [2^-600, 2^600] (so n, |xi| lie in
-  // [2^-300, 2^300]) and each numerator is zero or of magnitude >= 2^-300
-  // (their magnitudes are at most n or max(n, |xi|)).  A zero numerator
-  // gives a zero quotient whose sign may differ from the IEEE one; a zero of
-  // either sign adds nothing to F and compares equal in the break test, so
-  // the results are the same.
+  // Exact when xin2, n2 lie in [2^-600, 2^601) (so n, |xi| lie in
+  // [2^-300, 2^301)) and each eta_q is zero or of magnitude >= 2^-300:
+  // then every numerator is zero or of magnitude in [2^-352, 2^301] (at
+  // most n or max(n, |xi|)) over a denominator in [2^-300, 2^301), where
+  // the sequences need no scaling.  The numerator of s, n - |xi|, needs no
+  // test: n and |xi| are >= 2^-300, so both are multiples of 2^-352 and
+  // their difference is zero or at least 2^-352 in magnitude.  A zero
+  // numerator gives a zero quotient whose sign may differ from the IEEE
+  // one; a zero of either sign adds nothing to F and compares equal in the
+  // break test, so the results are the same.
   double xin = sqrt_nr(xin2), n = sqrt_nr(n2);
   const double dn = n - xin;
   double s = div_nr(dn, xin, recip_nr(xin));
   const double rn = recip_nr(n);
 #pragma unroll
   for (int q = 0; q < D; ++q) e[q] = div_nr(eta[q], n, rn);
-  int emin = __builtin_amdgcn_frexp_exp(dn); // zero -> 0, |x| >= 2^-300 -> >= -299
+  // zero -> 0, |x| >= 2^-300 -> >= -299
+  int emin = __builtin_amdgcn_frexp_exp(eta[0]);
 #pragma unroll
-  for (int q = 0; q < D; ++q) emin = min(emin, __builtin_amdgcn_frexp_exp(eta[q]));
+  for (int q = 1; q < D; ++q) emin = min(emin, __builtin_amdgcn_frexp_exp(eta[q]));
   const bool ok = sqrt_arg_in_range(xin2) && sqrt_arg_in_range(n2) && emin >= -299;
   if (live && !ok) { // outside the fast range: IEEE operators
     xin = sqrt(xin2);
