@@ -89,9 +89,10 @@ __device__ __forceinline__ bool pair_gate(const AdjArgs &B, uint32_t t, const Gr
 
 } // namespace
 
-// One node per ADJ_LANES consecutive lanes: lane r of node i takes slot r of
-// each column group (BGROUP == ADJ_LANES slots), contact entries r, r + L,
-// ... of i's list and wall sides r, r + L, ...; the lanes' partial
+// One node per L = ADJ_LANES consecutive lanes: lane r of node i takes slot
+// r % BGROUP of every (L/BGROUP)-th column group starting at group
+// r / BGROUP (slots r, r + L, ...), contact entries r, r + L, ... of i's
+// list and wall sides r, r + L, ...; the lanes' partial
 // cotangents are summed with shuffles and lane 0 writes the node's result.
 // The adjoint has no reference order to keep, so the sums run in any fixed
 // order.
@@ -99,7 +100,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_t t) {
   constexpr int N = 2 * D + 2; // dual directions: x_i (D), v_i (D), K, mu
   constexpr uint32_t L = ADJ_LANES;
-  static_assert(L == BGROUP, "one column-group slot per lane");
+  static_assert(L % BGROUP == 0 && (L & (L - 1)) == 0, "whole column groups per pass");
   using DN = Dual<N>;
   const StepArgs &A = B.A;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -143,9 +144,9 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
     {
       const uint32_t sl = i >> 6, lane = i & 63;
       const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-      const uint32_t *bc = A.bcol + base + BGROUP * lane + r;
-      const uint32_t *bk = A.break_step + base + BGROUP * lane + r;
-      for (uint32_t k0 = 0; k0 < width; k0 += BGROUP) {
+      const uint32_t *bc = A.bcol + base + BGROUP * lane + r % BGROUP;
+      const uint32_t *bk = A.break_step + base + BGROUP * lane + r % BGROUP;
+      for (uint32_t k0 = BGROUP * (r / BGROUP); k0 < width; k0 += L) {
         const uint32_t cw = bc[SLICE * k0], bks = bk[SLICE * k0];
         const bool live = cw != BCOL_PAD && bks >= t; // a bond broken at t is still used at t
         const uint32_t j = live ? (cw & BCOL_INDEX) : A.ghost;

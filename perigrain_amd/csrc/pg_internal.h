@@ -41,7 +41,7 @@ constexpr uint32_t BCOL_PAD = 0xFFFFFFFFu;
 
 constexpr int SLICE = 64;     // rows per sliced-ELL slice (= wavefront width)
 constexpr int BGROUP = 4;     // slots per lane per column load
-constexpr int ADJ_LANES = 4;  // lanes per node in the adjoint step kernel
+constexpr int ADJ_LANES = 4;  // lanes per node in the adjoint step kernel (8 measured no faster)
 constexpr int BCOL_TAIL = 512; // padding slots past the last slice (bond-loop read-ahead)
 
 // Slot k of the row in lane `lane` of a slice starting at `base`.
