@@ -17,6 +17,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -33,6 +35,20 @@ thread_local std::string g_err;
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
+}
+
+// Body of an exported entry point: no C++ exception leaves the C ABI; an
+// exception becomes an error code and message like any other failure.
+template <class F> int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return fail(PG_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(PG_ERR_INVALID, std::string("unexpected error: ") + e.what());
+  } catch (...) {
+    return fail(PG_ERR_INVALID, "unexpected error");
+  }
 }
 
 #define HIPCHK(x)                                                                  \
@@ -1254,10 +1270,12 @@ extern "C" {
 const char *pg_last_error(void) { return g_err.c_str(); }
 
 int pg_version(char *buf, size_t cap) {
-  const char *v = "libpg 0.1 (gfx950, FP64)";
-  if (!buf || cap == 0) return fail(PG_ERR_INVALID, "null buffer");
-  std::snprintf(buf, cap, "%s", v);
-  return PG_OK;
+  return guarded([&]() -> int {
+    const char *v = "libpg 0.1 (gfx950, FP64)";
+    if (!buf || cap == 0) return fail(PG_ERR_INVALID, "null buffer");
+    std::snprintf(buf, cap, "%s", v);
+    return PG_OK;
+  });
 }
 
 int pg_batch_create(const pg_setup *setups, const pg_params *params, int n, int device,
@@ -1290,457 +1308,503 @@ int pg_create(const pg_setup *setup, const pg_params *params, int device, pg_han
 }
 
 int pg_destroy(pg_handle *h) {
-  destroy_impl(h);
-  return PG_OK;
+  return guarded([&]() -> int {
+    destroy_impl(h);
+    return PG_OK;
+  });
 }
 
 int pg_step(pg_handle *h, uint32_t nsteps) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  HIPCHK(hipSetDevice(h->device));
-  return run_steps(h, nsteps);
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    return run_steps(h, nsteps);
+  });
 }
 
 int pg_sync(pg_handle *h) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PG_OK;
+  });
 }
 
 int pg_current_step(pg_handle *h, uint32_t *t) {
-  if (!h || !t) return fail(PG_ERR_INVALID, "null argument");
-  *t = h->t;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !t) return fail(PG_ERR_INVALID, "null argument");
+    *t = h->t;
+    return PG_OK;
+  });
 }
 
 int pg_sizes(pg_handle *h, int inst, uint64_t *n_nodes, uint64_t *n_grains, uint64_t *n_slots) {
-  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad handle or instance");
-  const InstHost &ih = h->inst[inst];
-  if (n_nodes) *n_nodes = ih.node_end - ih.node_begin;
-  if (n_grains) *n_grains = ih.grain_end - ih.grain_begin;
-  if (n_slots) *n_slots = ih.slot_end - ih.slot_begin;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad handle or instance");
+    const InstHost &ih = h->inst[inst];
+    if (n_nodes) *n_nodes = ih.node_end - ih.node_begin;
+    if (n_grains) *n_grains = ih.grain_end - ih.grain_begin;
+    if (n_slots) *n_slots = ih.slot_end - ih.slot_begin;
+    return PG_OK;
+  });
 }
 
 int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
-  if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  const int D = h->dim;
-  const InstHost &ih = h->inst[inst];
-  const size_t n = ih.node_end - ih.node_begin;
-  HIPCHK(hipSetDevice(h->device));
-  // device order -> reference order (the permutation stays inside the range)
-  auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
-  // component k of the instance's node range of a [D][ns] array
-  auto comp = [&](const double *base, int k, std::vector<double> &v) -> int {
-    v.resize(n);
-    return d2h(h, v.data(), base + (size_t)k * h->ns + ih.node_begin, n);
-  };
-  if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
-    // record parts [D][ns][2] of the instance's node range; CurrPos = pos + disp
-    std::vector<double> r((size_t)D * n * 2);
-    for (int q = 0; q < D; ++q)
-      RC(d2h(h, r.data() + (size_t)q * n * 2,
-             h->rec[h->t & 1] + rec_index(h->ns, ih.node_begin, 2 * q), n * 2));
-    auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
-    // pos of the grain moved by reset_partzero_y carries the y shift
-    const InstDev &id = h->inst_dev[inst];
-    size_t s0 = 0, s1 = 0;
-    if (id.yshift_grain >= 0) {
-      s0 = h->grain_node_begin[id.yshift_grain] - ih.node_begin;
-      s1 = h->grain_node_begin[id.yshift_grain + 1] - ih.node_begin;
-    }
-    for (size_t e = 0; e < n; ++e) {
-      const size_t a = at(e);
-      for (int k = 0; k < D; ++k) {
-        double X = el(a, k);
-        if (k == 1 && e >= s0 && e < s1) X = X + id.yshift;
-        dst[e * D + k] = field == PG_FIELD_CURRPOS ? X + el(a, D + k) : el(a, D + k);
+  return guarded([&]() -> int {
+    if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const int D = h->dim;
+    const InstHost &ih = h->inst[inst];
+    const size_t n = ih.node_end - ih.node_begin;
+    HIPCHK(hipSetDevice(h->device));
+    // device order -> reference order (the permutation stays inside the range)
+    auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
+    // component k of the instance's node range of a [D][ns] array
+    auto comp = [&](const double *base, int k, std::vector<double> &v) -> int {
+      v.resize(n);
+      return d2h(h, v.data(), base + (size_t)k * h->ns + ih.node_begin, n);
+    };
+    if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
+      // record parts [D][ns][2] of the instance's node range; CurrPos = pos + disp
+      std::vector<double> r((size_t)D * n * 2);
+      for (int q = 0; q < D; ++q)
+        RC(d2h(h, r.data() + (size_t)q * n * 2,
+               h->rec[h->t & 1] + rec_index(h->ns, ih.node_begin, 2 * q), n * 2));
+      auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
+      // pos of the grain moved by reset_partzero_y carries the y shift
+      const InstDev &id = h->inst_dev[inst];
+      size_t s0 = 0, s1 = 0;
+      if (id.yshift_grain >= 0) {
+        s0 = h->grain_node_begin[id.yshift_grain] - ih.node_begin;
+        s1 = h->grain_node_begin[id.yshift_grain + 1] - ih.node_begin;
       }
+      for (size_t e = 0; e < n; ++e) {
+        const size_t a = at(e);
+        for (int k = 0; k < D; ++k) {
+          double X = el(a, k);
+          if (k == 1 && e >= s0 && e < s1) X = X + id.yshift;
+          dst[e * D + k] = field == PG_FIELD_CURRPOS ? X + el(a, D + k) : el(a, D + k);
+        }
+      }
+      return PG_OK;
+    }
+    const double *src = nullptr;
+    switch (field) {
+      case PG_FIELD_VEL: src = h->vel[h->t & 1]; break;
+      case PG_FIELD_ACC: src = h->acc; break;
+      case PG_FIELD_FORCE: src = h->force_out; break;
+      default: return fail(PG_ERR_INVALID, "unknown field");
+    }
+    std::vector<double> v;
+    for (int k = 0; k < D; ++k) {
+      RC(comp(src, k, v));
+      for (size_t e = 0; e < n; ++e) dst[e * D + k] = v[at(e)];
     }
     return PG_OK;
-  }
-  const double *src = nullptr;
-  switch (field) {
-    case PG_FIELD_VEL: src = h->vel[h->t & 1]; break;
-    case PG_FIELD_ACC: src = h->acc; break;
-    case PG_FIELD_FORCE: src = h->force_out; break;
-    default: return fail(PG_ERR_INVALID, "unknown field");
-  }
-  std::vector<double> v;
-  for (int k = 0; k < D; ++k) {
-    RC(comp(src, k, v));
-    for (size_t e = 0; e < n; ++e) dst[e * D + k] = v[at(e)];
-  }
-  return PG_OK;
+  });
 }
 
 int pg_read_wall(pg_handle *h, int inst, double *wall_info, double *reaction) {
-  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  const int D = h->dim;
-  if (wall_info) RC(d2h(h, wall_info, h->wallpos[(h->t + 1) & 1] + (size_t)inst * 6, 6));
-  if (reaction) {
-    double r[18];
-    RC(d2h(h, r, h->reaction + (size_t)inst * 18, 18));
-    for (int s = 0; s < PG_MAX_SIDES; ++s)
-      for (int k = 0; k < D; ++k) reaction[s * D + k] = r[s * 3 + k];
-  }
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const int D = h->dim;
+    if (wall_info) RC(d2h(h, wall_info, h->wallpos[(h->t + 1) & 1] + (size_t)inst * 6, 6));
+    if (reaction) {
+      double r[18];
+      RC(d2h(h, r, h->reaction + (size_t)inst * 18, 18));
+      for (int s = 0; s < PG_MAX_SIDES; ++s)
+        for (int k = 0; k < D; ++k) reaction[s * D + k] = r[s * 3 + k];
+    }
+    return PG_OK;
+  });
 }
 
 int pg_grain_stats(pg_handle *h, int inst, double *out) {
-  if (!h || !out || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  const int D = h->dim;
-  const InstHost &ih = h->inst[inst];
-  const uint32_t ng = ih.grain_end - ih.grain_begin;
-  const size_t nv = (size_t)PG_GRAIN_STATS(D) * ng;
-  HIPCHK(hipSetDevice(h->device));
-  if (nv > h->stats_cap) {
-    RC(dfree(h, h->d_stats));
-    RC(dalloc(h, h->d_stats, nv));
-    h->stats_cap = nv;
-  }
-  HIPCHK(launch_grain_stats(D, h->A, h->t, ih.grain_begin, ng, h->d_stats, h->stream));
-  RC(d2h(h, out, h->d_stats, nv));
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !out || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const int D = h->dim;
+    const InstHost &ih = h->inst[inst];
+    const uint32_t ng = ih.grain_end - ih.grain_begin;
+    const size_t nv = (size_t)PG_GRAIN_STATS(D) * ng;
+    HIPCHK(hipSetDevice(h->device));
+    if (nv > h->stats_cap) {
+      RC(dfree(h, h->d_stats));
+      RC(dalloc(h, h->d_stats, nv));
+      h->stats_cap = nv;
+    }
+    HIPCHK(launch_grain_stats(D, h->A, h->t, ih.grain_begin, ng, h->d_stats, h->stream));
+    RC(d2h(h, out, h->d_stats, nv));
+    return PG_OK;
+  });
 }
 
 int pg_read_bond_intact(pg_handle *h, int inst, uint8_t *dst) {
-  if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  const std::vector<uint32_t> *stp = nullptr;
-  RC(bond_status(h, stp));
-  const std::vector<uint32_t> &st = *stp;
-  const InstHost &ih = h->inst[inst];
-  for (uint32_t e = ih.node_begin; e < ih.node_end; ++e) { // reference order
-    const uint32_t i = h->inv[e];
-    const uint32_t sl = i / SLICE, lane = i % SLICE;
-    for (uint32_t k = 0; k < h->deg[e]; ++k)
-      dst[h->slot0[e] - ih.slot_begin + k] =
-          (st[bslot(h->slice_base[sl], lane, k)] & BCOL_BROKEN) ? 0 : 1;
-  }
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const std::vector<uint32_t> *stp = nullptr;
+    RC(bond_status(h, stp));
+    const std::vector<uint32_t> &st = *stp;
+    const InstHost &ih = h->inst[inst];
+    for (uint32_t e = ih.node_begin; e < ih.node_end; ++e) { // reference order
+      const uint32_t i = h->inv[e];
+      const uint32_t sl = i / SLICE, lane = i % SLICE;
+      for (uint32_t k = 0; k < h->deg[e]; ++k)
+        dst[h->slot0[e] - ih.slot_begin + k] =
+            (st[bslot(h->slice_base[sl], lane, k)] & BCOL_BROKEN) ? 0 : 1;
+    }
+    return PG_OK;
+  });
 }
 
 int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_rows,
                  uint64_t *rows) {
-  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  const InstHost &ih = h->inst[inst];
-  if (grain < 0 || grain >= (int)(ih.grain_end - ih.grain_begin))
-    return fail(PG_ERR_INVALID, "bad grain");
-  const uint32_t gg = ih.grain_begin + grain;
-  const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
-  const std::vector<uint32_t> *stp = nullptr;
-  RC(bond_status(h, stp));
-  const std::vector<uint32_t> &st = *stp;
-  uint64_t r = 0;
-  for (uint32_t ge = a; ge < b; ++ge) { // reference node order
-    const uint32_t i = h->inv[ge];
-    const uint32_t sl = i / SLICE, lane = i % SLICE;
-    for (uint32_t k = 0; k < h->deg[ge]; ++k) {
-      const uint64_t e = bslot(h->slice_base[sl], lane, k);
-      if (st[e] & BCOL_BROKEN) continue;
-      const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
-      if (li < q) {
-        if (dst && r < cap_rows) {
-          dst[2 * r] = li;
-          dst[2 * r + 1] = q;
+  return guarded([&]() -> int {
+    if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const InstHost &ih = h->inst[inst];
+    if (grain < 0 || grain >= (int)(ih.grain_end - ih.grain_begin))
+      return fail(PG_ERR_INVALID, "bad grain");
+    const uint32_t gg = ih.grain_begin + grain;
+    const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
+    const std::vector<uint32_t> *stp = nullptr;
+    RC(bond_status(h, stp));
+    const std::vector<uint32_t> &st = *stp;
+    uint64_t r = 0;
+    for (uint32_t ge = a; ge < b; ++ge) { // reference node order
+      const uint32_t i = h->inv[ge];
+      const uint32_t sl = i / SLICE, lane = i % SLICE;
+      for (uint32_t k = 0; k < h->deg[ge]; ++k) {
+        const uint64_t e = bslot(h->slice_base[sl], lane, k);
+        if (st[e] & BCOL_BROKEN) continue;
+        const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
+        if (li < q) {
+          if (dst && r < cap_rows) {
+            dst[2 * r] = li;
+            dst[2 * r + 1] = q;
+          }
+          ++r;
         }
-        ++r;
       }
     }
-  }
-  if (rows) *rows = r;
-  return PG_OK;
+    if (rows) *rows = r;
+    return PG_OK;
+  });
 }
 
 int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs, uint64_t *pairs) {
-  if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-  if (h->t == 0) return fail(PG_ERR_STATE, "no step has been taken");
-  const int D = h->dim;
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemsetAsync(h->cl_cnt, 0, ((size_t)h->N + 1) * sizeof(uint32_t), h->stream));
-  HIPCHK(launch_pairs(D, false, h->A, h->t, h->cl_cnt, nullptr, nullptr, h->stream));
-  uint32_t *ptr = nullptr;
-  RC(dalloc(h, ptr, (size_t)h->N + 1));
-  int rc = scan(h, h->cl_cnt, ptr, (size_t)h->N + 1);
-  if (rc) {
-    dfree(h, ptr);
-    return rc;
-  }
-  uint32_t total = 0;
-  rc = d2h(h, &total, ptr + h->N, 1);
-  uint32_t *out = nullptr;
-  if (!rc) rc = dalloc(h, out, (size_t)2 * (total ? total : 1));
-  if (!rc) {
-    hipError_t e = launch_pairs(D, true, h->A, h->t, nullptr, ptr, out, h->stream);
-    if (e != hipSuccess) rc = fail(PG_ERR_DEVICE, hipGetErrorString(e));
-  }
-  std::vector<uint32_t> all((size_t)2 * total);
-  if (!rc) rc = d2h(h, all.data(), out, all.size());
-  dfree(h, ptr);
-  dfree(h, out);
-  if (rc) return rc;
-  const InstHost &ih = h->inst[inst];
-  // to reference indices, ordered by node and then by accumulation order
-  std::vector<std::pair<uint32_t, uint32_t>> pr;
-  for (uint64_t q = 0; q < total; ++q) {
-    const uint32_t i = h->perm[all[2 * q]], j = h->perm[all[2 * q + 1]];
-    if (i < ih.node_begin || i >= ih.node_end) continue;
-    pr.push_back({i - ih.node_begin, j - ih.node_begin});
-  }
-  std::stable_sort(pr.begin(), pr.end(),
-                   [](const std::pair<uint32_t, uint32_t> &x,
-                      const std::pair<uint32_t, uint32_t> &y) { return x.first < y.first; });
-  const uint64_t np = pr.size();
-  if (dst)
-    for (uint64_t q = 0; q < std::min(np, cap_pairs); ++q) {
-      dst[2 * q] = pr[q].first;
-      dst[2 * q + 1] = pr[q].second;
+  return guarded([&]() -> int {
+    if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    if (h->t == 0) return fail(PG_ERR_STATE, "no step has been taken");
+    const int D = h->dim;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemsetAsync(h->cl_cnt, 0, ((size_t)h->N + 1) * sizeof(uint32_t), h->stream));
+    HIPCHK(launch_pairs(D, false, h->A, h->t, h->cl_cnt, nullptr, nullptr, h->stream));
+    uint32_t *ptr = nullptr;
+    RC(dalloc(h, ptr, (size_t)h->N + 1));
+    int rc = scan(h, h->cl_cnt, ptr, (size_t)h->N + 1);
+    if (rc) {
+      dfree(h, ptr);
+      return rc;
     }
-  if (pairs) *pairs = np;
-  return PG_OK;
+    uint32_t total = 0;
+    rc = d2h(h, &total, ptr + h->N, 1);
+    uint32_t *out = nullptr;
+    if (!rc) rc = dalloc(h, out, (size_t)2 * (total ? total : 1));
+    if (!rc) {
+      hipError_t e = launch_pairs(D, true, h->A, h->t, nullptr, ptr, out, h->stream);
+      if (e != hipSuccess) rc = fail(PG_ERR_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<uint32_t> all((size_t)2 * total);
+    if (!rc) rc = d2h(h, all.data(), out, all.size());
+    dfree(h, ptr);
+    dfree(h, out);
+    if (rc) return rc;
+    const InstHost &ih = h->inst[inst];
+    // to reference indices, ordered by node and then by accumulation order
+    std::vector<std::pair<uint32_t, uint32_t>> pr;
+    for (uint64_t q = 0; q < total; ++q) {
+      const uint32_t i = h->perm[all[2 * q]], j = h->perm[all[2 * q + 1]];
+      if (i < ih.node_begin || i >= ih.node_end) continue;
+      pr.push_back({i - ih.node_begin, j - ih.node_begin});
+    }
+    std::stable_sort(pr.begin(), pr.end(),
+                     [](const std::pair<uint32_t, uint32_t> &x,
+                        const std::pair<uint32_t, uint32_t> &y) { return x.first < y.first; });
+    const uint64_t np = pr.size();
+    if (dst)
+      for (uint64_t q = 0; q < std::min(np, cap_pairs); ++q) {
+        dst[2 * q] = pr[q].first;
+        dst[2 * q + 1] = pr[q].second;
+      }
+    if (pairs) *pairs = np;
+    return PG_OK;
+  });
 }
 
 int pg_stream(pg_handle *h, void **stream) {
-  if (!h || !stream) return fail(PG_ERR_INVALID, "null argument");
-  *stream = (void *)h->stream;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !stream) return fail(PG_ERR_INVALID, "null argument");
+    *stream = (void *)h->stream;
+    return PG_OK;
+  });
 }
 
 int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries, uint64_t *skipped_steps) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  if (rebuilds) *rebuilds = h->rebuilds;
-  if (list_entries) *list_entries = h->list_entries;
-  if (skipped_steps) *skipped_steps = h->skipped;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    if (rebuilds) *rebuilds = h->rebuilds;
+    if (list_entries) *list_entries = h->list_entries;
+    if (skipped_steps) *skipped_steps = h->skipped;
+    return PG_OK;
+  });
 }
 
 int pg_set_tuning(pg_handle *h, double skin_factor, int block) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  if (!(skin_factor > 0) || skin_factor > 16)
-    return fail(PG_ERR_INVALID, "skin_factor must be in (0, 16]");
-  if (block != 64 && block != 128 && block != 256)
-    return fail(PG_ERR_INVALID, "block must be 64, 128 or 256");
-  HIPCHK(hipSetDevice(h->device));
-  h->block = block;
-  h->skin_factor = skin_factor;
-  h->args_gen++; // a captured chunk holds the old block size
-  for (uint32_t s = 0; s < h->I; ++s) {
-    InstHost &ih = h->inst[s];
-    ih.skin = skin_factor * (ih.Rc > 0 ? ih.Rc : 0.0);
-    const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
-    h->inst_dev[s].skin_half2 = sh * sh;
-  }
-  RC(h2d(h, h->insts, h->inst_dev.data(), h->I));
-  // the next step to run is t+1: rebuild from its positions
-  return build_list(h, h->t + 1);
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    if (!(skin_factor > 0) || skin_factor > 16)
+      return fail(PG_ERR_INVALID, "skin_factor must be in (0, 16]");
+    if (block != 64 && block != 128 && block != 256)
+      return fail(PG_ERR_INVALID, "block must be 64, 128 or 256");
+    HIPCHK(hipSetDevice(h->device));
+    h->block = block;
+    h->skin_factor = skin_factor;
+    h->args_gen++; // a captured chunk holds the old block size
+    for (uint32_t s = 0; s < h->I; ++s) {
+      InstHost &ih = h->inst[s];
+      ih.skin = skin_factor * (ih.Rc > 0 ? ih.Rc : 0.0);
+      const double sh = 0.5 * ih.skin * (1.0 - 1e-9);
+      h->inst_dev[s].skin_half2 = sh * sh;
+    }
+    RC(h2d(h, h->insts, h->inst_dev.data(), h->I));
+    // the next step to run is t+1: rebuild from its positions
+    return build_list(h, h->t + 1);
+  });
 }
 
 int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *block) {
-  if (!h || trial_steps == 0) return fail(PG_ERR_INVALID, "null handle or zero trial steps");
-  static const double skins[] = {0.25, 0.5, 1.0};
-  static const int blocks[] = {64, 128, 256};
-  double best = 1e300, best_skin = h->skin_factor;
-  int best_block = h->block;
-  for (double sk : skins)
-    for (int bl : blocks) {
-      RC(pg_set_tuning(h, sk, bl));
-      HIPCHK(hipStreamSynchronize(h->stream));
-      const auto t0 = std::chrono::steady_clock::now();
-      RC(run_steps(h, trial_steps));
-      HIPCHK(hipStreamSynchronize(h->stream));
-      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (dt < best) {
-        best = dt;
-        best_skin = sk;
-        best_block = bl;
+  return guarded([&]() -> int {
+    if (!h || trial_steps == 0) return fail(PG_ERR_INVALID, "null handle or zero trial steps");
+    static const double skins[] = {0.25, 0.5, 1.0};
+    static const int blocks[] = {64, 128, 256};
+    double best = 1e300, best_skin = h->skin_factor;
+    int best_block = h->block;
+    for (double sk : skins)
+      for (int bl : blocks) {
+        RC(pg_set_tuning(h, sk, bl));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        RC(run_steps(h, trial_steps));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt < best) {
+          best = dt;
+          best_skin = sk;
+          best_block = bl;
+        }
       }
-    }
-  RC(pg_set_tuning(h, best_skin, best_block));
-  if (skin_factor) *skin_factor = best_skin;
-  if (block) *block = best_block;
-  return PG_OK;
+    RC(pg_set_tuning(h, best_skin, best_block));
+    if (skin_factor) *skin_factor = best_skin;
+    if (block) *block = best_block;
+    return PG_OK;
+  });
 }
 
 int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
                        double *reaction_history) {
-  if (!h || !reaction_history) return fail(PG_ERR_INVALID, "null argument");
-  if (nsteps == 0 || checkpoint_every == 0) return fail(PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
-  if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
-  const InstDev &id = h->inst_dev[0];
-  if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped ||
-      h->inst[0].reset_step != 0 || (id.flags & IF_UNGATED))
-    return fail(PG_ERR_INVALID,
-                "the adjoint supports no torque, no timed movable/stoppable toggles, no clamped "
-                "nodes, no reset_partzero_y and only the standard loop");
-  HIPCHK(hipSetDevice(h->device));
-  auto &a = h->adj;
-  RC(adj_alloc(h, nsteps, checkpoint_every));
-  if (!h->wring) {
-    RC(dalloc(h, h->wring, (size_t)h->check_every * wring_stride(h)));
-    RC(dalloc(h, h->fring, (size_t)h->check_every * fring_stride(h)));
-    HIPCHK(hipMemsetAsync(h->fring, 0, (size_t)h->check_every * fring_stride(h), h->stream));
-  }
-  HIPCHK(launch_adj_init_break(h->d_bcol, a.break_step, h->E, h->stream));
-  a.t0 = h->t;
-  a.T = nsteps;
-  a.K = checkpoint_every;
-  a.recorded = false;
-  RC(ck_save(h, 0, h->t));
-  h->hist = a.hist;
-  h->hist_t0 = a.t0;
-  int rc = PG_OK;
-  for (uint32_t k = 0; k < a.nck && rc == PG_OK; ++k) {
-    const uint32_t n = std::min(a.K, a.T - k * a.K);
-    rc = run_steps(h, n);
-    if (rc == PG_OK) rc = ck_save(h, k + 1, h->t);
-  }
-  h->hist = nullptr;
-  RC(rc);
-  RC(d2d(h, a.fin_rec, h->rec[h->t & 1], rec_len(h)));
-  RC(d2d(h, a.fin_force, h->force_out, vec_len(h)));
-  RC(d2d(h, a.fin_reaction, h->reaction, (size_t)h->I * 18));
-  RC(d2d(h, a.fin_box, h->bbox[h->t % 3], box_len(h)));
-  const int D = h->dim;
-  std::vector<double> hh((size_t)nsteps * 18);
-  RC(d2h(h, hh.data(), a.hist, hh.size()));
-  for (size_t t = 0; t < nsteps; ++t)
-    for (int sd = 0; sd < PG_MAX_SIDES; ++sd)
-      for (int k = 0; k < D; ++k)
-        reaction_history[(t * PG_MAX_SIDES + sd) * D + k] = hh[t * 18 + sd * 3 + k];
-  a.recorded = true;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !reaction_history) return fail(PG_ERR_INVALID, "null argument");
+    if (nsteps == 0 || checkpoint_every == 0) return fail(PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
+    if (h->I != 1) return fail(PG_ERR_INVALID, "the differentiable form takes one instance per handle");
+    const InstDev &id = h->inst_dev[0];
+    if (h->any_torque || id.set_movable_grain >= 0 || id.set_stoppable_grain >= 0 || h->any_clamped ||
+        h->inst[0].reset_step != 0 || (id.flags & IF_UNGATED))
+      return fail(PG_ERR_INVALID,
+                  "the adjoint supports no torque, no timed movable/stoppable toggles, no clamped "
+                  "nodes, no reset_partzero_y and only the standard loop");
+    HIPCHK(hipSetDevice(h->device));
+    auto &a = h->adj;
+    RC(adj_alloc(h, nsteps, checkpoint_every));
+    if (!h->wring) {
+      RC(dalloc(h, h->wring, (size_t)h->check_every * wring_stride(h)));
+      RC(dalloc(h, h->fring, (size_t)h->check_every * fring_stride(h)));
+      HIPCHK(hipMemsetAsync(h->fring, 0, (size_t)h->check_every * fring_stride(h), h->stream));
+    }
+    HIPCHK(launch_adj_init_break(h->d_bcol, a.break_step, h->E, h->stream));
+    a.t0 = h->t;
+    a.T = nsteps;
+    a.K = checkpoint_every;
+    a.recorded = false;
+    RC(ck_save(h, 0, h->t));
+    h->hist = a.hist;
+    h->hist_t0 = a.t0;
+    int rc = PG_OK;
+    for (uint32_t k = 0; k < a.nck && rc == PG_OK; ++k) {
+      const uint32_t n = std::min(a.K, a.T - k * a.K);
+      rc = run_steps(h, n);
+      if (rc == PG_OK) rc = ck_save(h, k + 1, h->t);
+    }
+    h->hist = nullptr;
+    RC(rc);
+    RC(d2d(h, a.fin_rec, h->rec[h->t & 1], rec_len(h)));
+    RC(d2d(h, a.fin_force, h->force_out, vec_len(h)));
+    RC(d2d(h, a.fin_reaction, h->reaction, (size_t)h->I * 18));
+    RC(d2d(h, a.fin_box, h->bbox[h->t % 3], box_len(h)));
+    const int D = h->dim;
+    std::vector<double> hh((size_t)nsteps * 18);
+    RC(d2h(h, hh.data(), a.hist, hh.size()));
+    for (size_t t = 0; t < nsteps; ++t)
+      for (int sd = 0; sd < PG_MAX_SIDES; ++sd)
+        for (int k = 0; k < D; ++k)
+          reaction_history[(t * PG_MAX_SIDES + sd) * D + k] = hh[t * 18 + sd * 3 + k];
+    a.recorded = true;
+    return PG_OK;
+  });
 }
 
 int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad) {
-  if (!h || !reaction_bar || !grad) return fail(PG_ERR_INVALID, "null argument");
-  auto &a = h->adj;
-  if (!a.recorded) return fail(PG_ERR_STATE, "no recorded forward pass (pg_adjoint_forward)");
-  HIPCHK(hipSetDevice(h->device));
-  const int D = h->dim;
-  RC(h2d(h, a.lR, reaction_bar, (size_t)a.T * PG_MAX_SIDES * D));
-  HIPCHK(hipMemsetAsync(a.Pu, 0, vec_len(h) * sizeof(double), h->stream));
-  HIPCHK(hipMemsetAsync(a.Pv, 0, vec_len(h) * sizeof(double), h->stream));
-  HIPCHK(hipMemsetAsync(a.Pa, 0, vec_len(h) * sizeof(double), h->stream));
-  HIPCHK(hipMemsetAsync(a.lf[0], 0, vec_len(h) * sizeof(double), h->stream));
-  HIPCHK(hipMemsetAsync(a.lf[1], 0, vec_len(h) * sizeof(double), h->stream));
-  const uint32_t blocks = (uint32_t)(((size_t)h->N * ADJ_LANES + 255) / 256);
-  HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
-  // X parts of the segment's record slots (the recompute writes only u)
-  for (uint32_t q = 0; q < std::min(a.K, a.T); ++q) RC(d2d(h, a.su + (size_t)q * rec_len(h), h->rec[0], rec_len(h)));
-  InstHost &ih = h->inst[0];
-  const double skin0 = ih.skin;
-  int rc = PG_OK;
-  for (int64_t k = (int64_t)a.nck - 1; k >= 0 && rc == PG_OK; --k) {
-    const uint32_t c = a.t0 + (uint32_t)k * a.K;
-    const uint32_t n = std::min(a.K, a.T - (uint32_t)k * a.K);
-    rc = ck_restore(h, (uint32_t)k, c);
-    if (rc == PG_OK) rc = adj_recompute(h, c, n);
-    if (rc != PG_OK) break;
-    // one candidate list for the whole segment: pairs within the contact
-    // radius at any of its steps are within radius + twice the largest
-    // displacement at its first step
-    const unsigned long long zero = 0;
-    HIPCHK(hipMemcpyAsync(a.dmax, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(launch_adj_maxdisp(D, h->A, a.su, n, a.dmax, h->stream));
-    unsigned long long dk = 0;
-    RC(d2h(h, &dk, a.dmax, 1));
-    const double dmax = dkey_inv(dk);
-    ih.skin = 2.0 * dmax * (1.0 + 1e-6) + 1e-12 * ih.Rc;
-    rc = build_list(h, c + 1, a.su);
-    ih.skin = skin0;
-    if (rc != PG_OK) break;
-    for (uint32_t t = c + n; t >= c + 1; --t) {
-      const size_t q = t - c - 1;
-      AdjArgs B;
-      B.A = h->A;
-      B.su = a.su + q * rec_len(h);
-      B.sv = a.sv + q * vec_len(h);
-      B.swall = a.swall + q * h->I * 6;
-      B.sbox = a.sbox + q * box_len(h);
-      B.lR = a.lR + (size_t)(t - a.t0 - 1) * PG_MAX_SIDES * D;
-      B.lf_in = a.lf[t & 1];
-      B.lf_out = a.lf[(t - 1) & 1];
-      B.Pu = a.Pu;
-      B.Pv = a.Pv;
-      B.Pa = a.Pa;
-      B.gpart = a.gpart;
-      hipError_t e = launch_adj_step(D, B, t, blocks, h->stream);
-      if (e != hipSuccess) {
-        rc = fail(PG_ERR_DEVICE, std::string("adjoint step: ") + hipGetErrorString(e));
-        break;
+  return guarded([&]() -> int {
+    if (!h || !reaction_bar || !grad) return fail(PG_ERR_INVALID, "null argument");
+    auto &a = h->adj;
+    if (!a.recorded) return fail(PG_ERR_STATE, "no recorded forward pass (pg_adjoint_forward)");
+    HIPCHK(hipSetDevice(h->device));
+    const int D = h->dim;
+    RC(h2d(h, a.lR, reaction_bar, (size_t)a.T * PG_MAX_SIDES * D));
+    HIPCHK(hipMemsetAsync(a.Pu, 0, vec_len(h) * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(a.Pv, 0, vec_len(h) * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(a.Pa, 0, vec_len(h) * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(a.lf[0], 0, vec_len(h) * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(a.lf[1], 0, vec_len(h) * sizeof(double), h->stream));
+    const uint32_t blocks = (uint32_t)(((size_t)h->N * ADJ_LANES + 255) / 256);
+    HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
+    // X parts of the segment's record slots (the recompute writes only u)
+    for (uint32_t q = 0; q < std::min(a.K, a.T); ++q) RC(d2d(h, a.su + (size_t)q * rec_len(h), h->rec[0], rec_len(h)));
+    InstHost &ih = h->inst[0];
+    const double skin0 = ih.skin;
+    int rc = PG_OK;
+    for (int64_t k = (int64_t)a.nck - 1; k >= 0 && rc == PG_OK; --k) {
+      const uint32_t c = a.t0 + (uint32_t)k * a.K;
+      const uint32_t n = std::min(a.K, a.T - (uint32_t)k * a.K);
+      rc = ck_restore(h, (uint32_t)k, c);
+      if (rc == PG_OK) rc = adj_recompute(h, c, n);
+      if (rc != PG_OK) break;
+      // one candidate list for the whole segment: pairs within the contact
+      // radius at any of its steps are within radius + twice the largest
+      // displacement at its first step
+      const unsigned long long zero = 0;
+      HIPCHK(hipMemcpyAsync(a.dmax, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+      HIPCHK(launch_adj_maxdisp(D, h->A, a.su, n, a.dmax, h->stream));
+      unsigned long long dk = 0;
+      RC(d2h(h, &dk, a.dmax, 1));
+      const double dmax = dkey_inv(dk);
+      ih.skin = 2.0 * dmax * (1.0 + 1e-6) + 1e-12 * ih.Rc;
+      rc = build_list(h, c + 1, a.su);
+      ih.skin = skin0;
+      if (rc != PG_OK) break;
+      for (uint32_t t = c + n; t >= c + 1; --t) {
+        const size_t q = t - c - 1;
+        AdjArgs B;
+        B.A = h->A;
+        B.su = a.su + q * rec_len(h);
+        B.sv = a.sv + q * vec_len(h);
+        B.swall = a.swall + q * h->I * 6;
+        B.sbox = a.sbox + q * box_len(h);
+        B.lR = a.lR + (size_t)(t - a.t0 - 1) * PG_MAX_SIDES * D;
+        B.lf_in = a.lf[t & 1];
+        B.lf_out = a.lf[(t - 1) & 1];
+        B.Pu = a.Pu;
+        B.Pv = a.Pv;
+        B.Pa = a.Pa;
+        B.gpart = a.gpart;
+        hipError_t e = launch_adj_step(D, B, t, blocks, h->stream);
+        if (e != hipSuccess) {
+          rc = fail(PG_ERR_DEVICE, std::string("adjoint step: ") + hipGetErrorString(e));
+          break;
+        }
       }
     }
-  }
-  ih.skin = skin0;
-  RC(rc);
-  std::vector<double> gp((size_t)blocks * 3);
-  RC(d2h(h, gp.data(), a.gpart, gp.size()));
-  grad[0] = grad[1] = grad[2] = 0.0;
-  for (uint32_t b = 0; b < blocks; ++b)
-    for (int q = 0; q < 3; ++q) grad[q] += gp[(size_t)b * 3 + q];
-  // back to the state after the recorded pass
-  RC(ck_restore(h, a.nck, a.t0 + a.T));
-  RC(d2d(h, h->rec[h->t & 1], a.fin_rec, rec_len(h)));
-  RC(d2d(h, h->force_out, a.fin_force, vec_len(h)));
-  RC(d2d(h, h->reaction, a.fin_reaction, (size_t)h->I * 18));
-  RC(d2d(h, h->bbox[h->t % 3], a.fin_box, box_len(h)));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return PG_OK;
+    ih.skin = skin0;
+    RC(rc);
+    std::vector<double> gp((size_t)blocks * 3);
+    RC(d2h(h, gp.data(), a.gpart, gp.size()));
+    grad[0] = grad[1] = grad[2] = 0.0;
+    for (uint32_t b = 0; b < blocks; ++b)
+      for (int q = 0; q < 3; ++q) grad[q] += gp[(size_t)b * 3 + q];
+    // back to the state after the recorded pass
+    RC(ck_restore(h, a.nck, a.t0 + a.T));
+    RC(d2d(h, h->rec[h->t & 1], a.fin_rec, rec_len(h)));
+    RC(d2d(h, h->force_out, a.fin_force, vec_len(h)));
+    RC(d2d(h, h->reaction, a.fin_reaction, (size_t)h->I * 18));
+    RC(d2d(h, h->bbox[h->t % 3], a.fin_box, box_len(h)));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PG_OK;
+  });
 }
 
 int pg_set_graph(pg_handle *h, int use_graph) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  h->use_graph = use_graph != 0;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    h->use_graph = use_graph != 0;
+    return PG_OK;
+  });
 }
 
 // Events on the handle's stream, for timing by the caller.
 int pg_event_record(pg_handle *h, int slot) {
-  if (!h || slot < 0 || slot >= 8) return fail(PG_ERR_INVALID, "bad event slot");
-  HIPCHK(hipEventRecord(h->ev[slot], h->stream));
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || slot < 0 || slot >= 8) return fail(PG_ERR_INVALID, "bad event slot");
+    HIPCHK(hipEventRecord(h->ev[slot], h->stream));
+    return PG_OK;
+  });
 }
 
 int pg_event_elapsed(pg_handle *h, int a, int b, float *ms) {
-  if (!h || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(PG_ERR_INVALID, "bad event slot");
-  HIPCHK(hipEventSynchronize(h->ev[b]));
-  HIPCHK(hipEventElapsedTime(ms, h->ev[a], h->ev[b]));
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return fail(PG_ERR_INVALID, "bad event slot");
+    HIPCHK(hipEventSynchronize(h->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, h->ev[a], h->ev[b]));
+    return PG_OK;
+  });
 }
 
 int pg_kernel_time(pg_handle *h, double *ms, uint64_t *launches) {
-  if (!h) return fail(PG_ERR_INVALID, "null handle");
-  if (ms) *ms = h->kernel_ms;
-  if (launches) *launches = h->kernel_launches;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(PG_ERR_INVALID, "null handle");
+    if (ms) *ms = h->kernel_ms;
+    if (launches) *launches = h->kernel_launches;
+    return PG_OK;
+  });
 }
 
 } // extern "C"
 
 extern "C" int pg_device_count(int *n) {
-  if (!n) return fail(PG_ERR_INVALID, "null argument");
-  *n = 0;
-  int c = 0;
-  hipError_t e = hipGetDeviceCount(&c);
-  if (e == hipErrorNoDevice) return PG_OK;
-  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
-  *n = c;
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!n) return fail(PG_ERR_INVALID, "null argument");
+    *n = 0;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e == hipErrorNoDevice) return PG_OK;
+    if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    *n = c;
+    return PG_OK;
+  });
 }
 
 extern "C" int pg_check_arith(int device, uint64_t n, const double *a, const double *b, double *q,
                               double *r) {
-  if (!a || !b || !q || !r) return fail(PG_ERR_INVALID, "null argument");
-  HIPCHK(hipSetDevice(device));
-  double *d = nullptr;
-  HIPCHK(hipMalloc(&d, 4 * (n ? n : 1) * sizeof(double)));
-  const size_t bytes = n * sizeof(double);
-  hipError_t e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_arith(n, d, d + n, d + 2 * n, d + 3 * n, nullptr);
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = hipMemcpy(q, d + 2 * n, bytes, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(r, d + 3 * n, bytes, hipMemcpyDeviceToHost);
-  (void)hipFree(d);
-  if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("pg_check_arith: ") + hipGetErrorString(e));
-  return PG_OK;
+  return guarded([&]() -> int {
+    if (!a || !b || !q || !r) return fail(PG_ERR_INVALID, "null argument");
+    HIPCHK(hipSetDevice(device));
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, 4 * (n ? n : 1) * sizeof(double)));
+    const size_t bytes = n * sizeof(double);
+    hipError_t e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_arith(n, d, d + n, d + 2 * n, d + 3 * n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(q, d + 2 * n, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(r, d + 3 * n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("pg_check_arith: ") + hipGetErrorString(e));
+    return PG_OK;
+  });
 }
