@@ -3,6 +3,7 @@ workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
 moving top wall, gravity), one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-adjoint]
+                    [--no-configs]
     python bench.py --copy-calibration MIB   # plain device copy, for PMC scaling
 
 Prints one JSON line.  `value` is node-steps/s with the state resident in
@@ -23,6 +24,10 @@ grains (~15k nodes, 2000 steps, checkpoints every 100): the plain forward
 run, the recorded forward pass and the adjoint (backward) pass, each timed on
 the host clock after a warm-up run, and the adjoint gradient of the top-wall
 reaction misfit against central finite differences (relative step 1e-6).
+
+`other_workloads` times the two 2D workloads of the measurement plan that
+fit one GPU beside the headline one (the reference's two-disk collision and
+the 400-grain bulk settle) on the GPU, and the oracle on a sample of each.
 
 `batched` measures the batched form on 256 independent 2D gravity-settle
 instances (~2.3M nodes, uneven sizes): one handle holding all of them
@@ -206,6 +211,48 @@ def adjoint_bench(steps=2000, ck=100):
     }
 
 
+def secondary_bench(threads, with_cpu):
+    """The other single-GPU workloads of the measurement plan: the
+    reference's own two-disk collision (2D, ~1.4k nodes, its full 5000
+    steps) and the 2D bulk settle of 400 grains (~122k nodes, 2000 steps),
+    each timed on the host clock over whole pg_step calls, and the oracle
+    on a bounded sample of each with `threads` threads and with one."""
+    from perigrain_amd import Stepper, setups
+    out = {}
+    cases = (
+        ("two_disk_collision_2d", lambda: setups.two_disk_collision(seed=0), 5000, 10, 1000),
+        ("bulk_settle_2d_400_grains", lambda: setups.bulk_settle_2d(seed=0), 2000, 1, 20),
+    )
+    for name, make, gpu_steps, cpu_warm, cpu_steps in cases:
+        s, p = make()
+        with Stepper(s, p, device=0) as g:  # warm-up handle
+            g.step(10)
+            g.sync()
+        with Stepper(s, p, device=0) as g:
+            g.sync()
+            t0 = time.perf_counter()
+            g.step(gpu_steps)
+            g.sync()
+            dt = time.perf_counter() - t0
+        rec = {"nodes": s.n_nodes, "directed_bonds": s.n_slots(), "gpu_steps": gpu_steps,
+               "gpu_ms_per_step": dt * 1e3 / gpu_steps,
+               "gpu_node_steps_per_s": s.n_nodes * gpu_steps / dt}
+        if with_cpu:
+            import oracle
+            for th in sorted({threads, 1}, reverse=True):
+                o = oracle.Oracle(s, p, threads=th)
+                o.step(cpu_warm)
+                t0 = time.perf_counter()
+                o.step(cpu_steps)
+                dtc = time.perf_counter() - t0
+                o.close()
+                rec[f"cpu_node_steps_per_s_{th}_threads"] = s.n_nodes * cpu_steps / dtc
+            rec["cpu_sample"] = f"oracle, {cpu_steps} steps after {cpu_warm} warm-up steps"
+        print(f"bench: {name}: {rec}", file=sys.stderr, flush=True)
+        out[name] = rec
+    return out
+
+
 def measured_traffic(workload, n_nodes, n_slots):
     """HBM bytes per step-kernel launch from the committed PMC summary, if
     it was measured on this exact workload; else None."""
@@ -229,6 +276,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batched-steps", type=int, default=512)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 2D two-disk and bulk-settle workloads")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -287,9 +336,13 @@ def main():
     if not args.no_adjoint:
         adj = adjoint_bench()
 
+    threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+    others = None
+    if not args.no_configs:
+        others = secondary_bench(threads, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
         cpu = cpu_baseline(setup, params, threads)
 
     line = {
@@ -328,6 +381,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adj,
+        "other_workloads": others,
     }
     print(json.dumps(line), flush=True)
 
