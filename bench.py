@@ -287,8 +287,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(see DESIGN.md for how the stepper would shard).", file=sys.stderr)
+        print(f"bench.py: the stepper drives one GPU; --gpus {args.gpus} is not supported.",
+              file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: multi-rank launch is not supported; run with one process.",
