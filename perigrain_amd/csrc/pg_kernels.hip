@@ -224,6 +224,104 @@ __device__ __forceinline__ void wave_box_atomic(unsigned long long *keys, uint32
   }
 }
 
+// Grain boxes combined per workgroup first: the boxes of the workgroup's
+// first BOX_SLOTS grains (nodes are grain-ordered, so a workgroup's grains
+// are consecutive from its first node's grain g0) collect in LDS, and the
+// workgroup's last wavefront to finish adds them to the global keys, so a
+// key sees one global atomic per workgroup instead of one per wavefront.
+// Grains beyond the slots go straight to the global keys.
+constexpr uint32_t BOX_SLOTS = 4;
+struct BlockBox {
+  unsigned long long key[BOX_SLOTS][6];
+  uint32_t g0;   // grain of the workgroup's first node (0xFFFFFFFF: none)
+  uint32_t done; // wavefronts that have added their boxes
+};
+
+// Set up by every thread of the workgroup, followed by a barrier.
+template <int D>
+__device__ __forceinline__ void block_box_init(BlockBox &bb, const StepArgs &A,
+                                               uint32_t first_node) {
+  if (threadIdx.x < BOX_SLOTS * 2 * D) {
+    const uint32_t k = threadIdx.x % (2 * D);
+    bb.key[threadIdx.x / (2 * D)][k] = k < (uint32_t)D ? ~0ull : 0ull;
+  }
+  if (threadIdx.x == 0) {
+    bb.g0 = first_node < A.n_nodes ? A.node_grain[first_node] : 0xFFFFFFFFu;
+    bb.done = 0;
+  }
+}
+
+// wave_box_atomic's reduction, then the workgroup combination; all lanes of
+// every wavefront of the workgroup call it.
+template <int D>
+__device__ __forceinline__ void wave_box_block(unsigned long long *keys, uint32_t grp, bool valid,
+                                               const double *x, BlockBox &bb) {
+  unsigned long long key[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) key[k] = dkey(x[k]);
+  const bool lane0 = (threadIdx.x & 63) == 0;
+  bool todo = valid && grp != 0xFFFFFFFFu;
+  for (;;) {
+    const unsigned long long m = __ballot(todo);
+    if (m == 0) break;
+    const uint32_t g0 = __builtin_amdgcn_readlane(grp, (int)__builtin_ctzll(m));
+    const bool in = todo && grp == g0;
+    unsigned long long kmin[D], kmax[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      kmin[k] = in ? key[k] : ~0ull;
+      kmax[k] = in ? key[k] : 0ull;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const unsigned long long a = __shfl_xor(kmin[k], off, 64);
+        const unsigned long long b = __shfl_xor(kmax[k], off, 64);
+        kmin[k] = a < kmin[k] ? a : kmin[k];
+        kmax[k] = b > kmax[k] ? b : kmax[k];
+      }
+    }
+    if (lane0) {
+      const uint32_t slot = g0 - bb.g0; // g0 >= bb.g0: the first node's grain is the smallest
+      if (slot < BOX_SLOTS) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          atomicMin(&bb.key[slot][k], kmin[k]);
+          atomicMax(&bb.key[slot][D + k], kmax[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          atomicMin(&keys[(size_t)g0 * 2 * D + k], kmin[k]);
+          atomicMax(&keys[(size_t)g0 * 2 * D + D + k], kmax[k]);
+        }
+      }
+    }
+    todo = todo && !in;
+  }
+  // this wavefront's LDS atomics are done before it counts itself; the
+  // wavefront that counts last sees every other wavefront's
+  __threadfence_block();
+  uint32_t prev = 0;
+  if (lane0) prev = atomicAdd(&bb.done, 1u);
+  prev = __builtin_amdgcn_readfirstlane(prev);
+  if (prev + 1 == blockDim.x / 64) {
+    __threadfence_block();
+    const uint32_t l = threadIdx.x & 63;
+    if (l < BOX_SLOTS * 2 * D && bb.g0 != 0xFFFFFFFFu) {
+      const uint32_t slot = l / (2 * D), k = l % (2 * D);
+      const unsigned long long v = bb.key[slot][k];
+      // an untouched slot holds the identity, whose atomic would change nothing
+      if (k < (uint32_t)D) {
+        if (v != ~0ull) atomicMin(&keys[(size_t)(bb.g0 + slot) * 2 * D + k], v);
+      } else {
+        if (v != 0ull) atomicMax(&keys[(size_t)(bb.g0 + slot) * 2 * D + k], v);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ step kernel
 
 // One full time step t for every node (timeloop.h:608-878):
@@ -298,7 +396,7 @@ __device__ __forceinline__ void step_housekeeping(const StepArgs &A, uint32_t t,
 // boxes are combined across the wavefront).
 template <int D, bool RECORD, bool UVOL, class RS>
 __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, const uint32_t i,
-                                          const RS &rs) {
+                                          const RS &rs, BlockBox *bb = nullptr) {
   const uint32_t ns = A.ns;
   const bool valid = i < A.n_nodes;
   uint32_t g = 0xFFFFFFFFu;
@@ -571,7 +669,8 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
   }
 
   if (viol) atomicMin(A.violation, t + 1);
-  wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
+  if (bb) wave_box_block<D>(A.bbox[(t + 1) % 3], g, valid, xn, *bb);
+  else wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
 }
 
 template <int D, bool RECORD, bool UVOL>
@@ -582,8 +681,12 @@ __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepAr
   const uint32_t nb = gridDim.x;
   const uint32_t lb = xcd_block(blockIdx.x, nb);
   const uint32_t i = lb * blockDim.x + threadIdx.x;
+  __shared__ BlockBox bb;
+  block_box_init<D>(bb, A, lb * blockDim.x);
+  __syncthreads();
   step_housekeeping<D>(A, t, i, nb * blockDim.x);
-  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>(A.rec[t & 1], A.ns, A.V, A.ghost));
+  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>(A.rec[t & 1], A.ns, A.V, A.ghost),
+                             &bb);
 }
 
 // The same step with each workgroup's partner window staged in LDS: block b
