@@ -48,3 +48,32 @@ def test_create_rejects_bad_setup():
     s2.conn[5, 1] = s2.node_offset[1] + 7  # index beyond grain 0
     with pytest.raises(_ffi.PgError, match="out of range"):
         Stepper(s2, p2, device=0)
+
+
+def test_libpgio_exports_every_declared_symbol():
+    from perigrain_amd import h5io
+    lib = ctypes.CDLL(h5io.LIBPGIO_PATH)
+    missing = [n for n in declared("pgio.h") if not hasattr(lib, n)]
+    assert declared("pgio.h") and not missing, f"libpgio.so lacks {missing}"
+
+
+def test_create_rejects_bad_loop_settings():
+    import pytest
+    from perigrain_amd import Stepper, setups
+    s, p = setups.two_disk_collision(seed=0)
+    p.timeloop_variant = 3
+    with pytest.raises(_ffi.PgError, match="timeloop_variant"):
+        Stepper(s, p, device=0)
+    s, p = setups.two_disk_collision(seed=0)
+    s = setups.subset(s, [0])
+    p.reset_partzero_y, p.reset_partzero_y_timestep = 1, 10
+    with pytest.raises(_ffi.PgError, match="second grain"):
+        Stepper(s, p, device=0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
