@@ -45,6 +45,7 @@ def _assert_close(g, o, rtol, what=""):
         a, b = g.read(f), o.read(f)
         scale = max(np.max(np.abs(b)), 1e-300)
         err = np.max(np.abs(a - b)) / scale
+        print(f"{what} {f}: max error {err:.3e} relative to max |x|", flush=True)
         assert err <= rtol, f"{what} {f}: max error {err:.3e} (relative to max |x|) > {rtol}"
     assert np.array_equal(g.bond_intact(), o.bond_intact()), f"{what} bond status differs"
     assert np.array_equal(g.contacts(), o.contacts()), f"{what} contact pairs differ"
@@ -197,7 +198,7 @@ def test_2d_settle_with_walls_close(oracle_threads):
         s.vel[s.grain_nodes(gi)] = 100.0 * np.array([np.cos(a), np.sin(a)])
     g, o = _both(s, p, oracle_threads)
     done = 0
-    for t in (1, 200, 600):
+    for t in (1, 200, 600, 1000):
         _advance(g, o, t - done)
         done = t
         _assert_close(g, o, 1e-10, f"t={t}")
