@@ -93,3 +93,23 @@ def test_gradient_matches_cpu_dual_number_reference(oracle_threads):
     raw_cpu = np.einsum("tsk,tskq->q", bar_c, dR)
     print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
     np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)
+
+
+def test_3d_adjoint_matches_cpu_dual_numbers(oracle_threads):
+    # 8 spheres pressed by the top wall (z_max, side 5) from the first step;
+    # 100 steps in segments of 30 (the last one shorter)
+    import oracle
+    from perigrain_amd import Stepper, setups
+    s, p = setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-50.0)
+    steps, ck = 100, 30
+    R, dR = oracle.reaction_history_tangent(s, p, steps, threads=oracle_threads)
+    with Stepper(s, p, device=0) as g:
+        hist = g.adjoint_forward(steps, ck)
+        assert np.array_equal(hist, R), "the recorded 3D reaction history is bit-exact"
+        assert np.all(np.any(hist[:, 5] != 0, axis=1)), "the top wall should push at every step"
+        bar = np.zeros_like(hist)
+        bar[:, 5] = 2.0 * hist[:, 5]  # misfit against a zero target
+        raw = g.adjoint_backward(bar)
+    raw_cpu = np.einsum("tsk,tskq->q", bar, dR)
+    print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
+    np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)

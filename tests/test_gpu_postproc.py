@@ -61,3 +61,19 @@ def test_grain_stats_3d_compression():
     with Stepper(s, p, device=0) as g:
         g.step(120)
         _check(g.grain_stats(), _numpy_stats(g, s), 3)
+
+
+def test_grain_stats_of_batched_instances():
+    # each instance of a batched handle reports its own grains, as a
+    # single-instance handle of the same setup does
+    sp = [setups.settle_instance_2d(seed) for seed in (21, 22)]
+    ss = [setups.subset(s, list(range(min(10, s.n_grains)))) for s, _ in sp]
+    ps = [p for _, p in sp]
+    with Stepper(ss, ps, device=0) as b:
+        b.step(80)
+        for k, (s, p) in enumerate(zip(ss, ps)):
+            st = b.grain_stats(inst=k)
+            assert st["force_sum"].shape == (s.n_grains, 2)
+            with Stepper(s, p, device=0) as one:
+                one.step(80)
+                _check(st, _numpy_stats(one, s), 2)
