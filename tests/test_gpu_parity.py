@@ -93,7 +93,7 @@ def test_two_disk_fracture_bit_exact(oracle_threads):
     s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
     g, o = _both(s, p, oracle_threads)
     done = 0
-    for t in (20, 100, 250):
+    for t in (20, 100, 250, 1000):
         _advance(g, o, t - done)
         done = t
         _assert_exact(g, o, f"t={t}")
@@ -114,6 +114,20 @@ def test_3d_compression_with_walls_bit_exact(oracle_threads, monkeypatch, lds):
         _assert_exact(g, o, f"t={t}")
     _, r = o.wall()
     assert np.any(r != 0), "top wall should push"
+
+
+def test_3d_compression_long_horizon_bit_exact(oracle_threads):
+    # 2000 steps of 8 spheres under a top wall moving at 20 m/s: contact
+    # lists change as the pack is pressed together
+    s, p = setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-20.0)
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (500, 1000, 2000):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+    _, r = o.wall()
+    assert r[5, 2] > 0, "top wall should push"
 
 
 def _falling_disk(seed, reset_step):
