@@ -77,3 +77,11 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_null_handle_is_an_error_without_a_device():
+    lib = _ffi.lib()
+    invalid = 1  # PG_ERR_INVALID
+    assert lib.pg_step(None, 1) == invalid
+    assert b"null handle" in lib.pg_last_error()
+    assert lib.pg_sync(None) == invalid
