@@ -575,3 +575,20 @@ def test_tangent_mode_matches_central_differences():
             vals.append(_raw_loss(sq, p, steps, target))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(grad[q] - fd) <= 1e-5 * abs(fd), (q, grad[q], fd)
+
+
+def test_results_do_not_depend_on_the_thread_count():
+    # the oracle runs grains on OpenMP threads; every sum keeps its own
+    # order, so one thread and several give the same bits
+    s, p = setups.two_disk_collision(seed=4, gap=1e-5, speed=50.0)
+    nodes = s.grain_nodes(0)
+    s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
+    out = []
+    for threads in (1, 4):
+        o = oracle.Oracle(s, p, threads=threads)
+        o.step(60, record_contacts=True)
+        out.append((o.read("CurrPos"), o.read("force"), o.bond_intact(), o.contacts()))
+        o.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    assert (out[0][2] == 0).sum() > 0
