@@ -3,12 +3,14 @@ workload (64 spheres, ~265k nodes, ~23.5M directed bonds, fracture on,
 moving top wall, gravity), one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-adjoint]
-                    [--no-configs]
+                    [--no-configs] [--tuning SKIN,BLOCK]
     python bench.py --copy-calibration MIB   # plain device copy, for PMC scaling
 
 Prints one JSON line.  `value` is node-steps/s with the state resident in
 HBM; the timed region is K steps enqueued through pg_step, bracketed by HIP
-events on the stepper's own stream.  `roofline` prices the fused step
+events on the stepper's own stream.  The Verlet skin and the block size come
+from the project's autotuner (pg_autotune), run first on a separate handle
+of the same workload; both are result-neutral.  `roofline` prices the fused step
 kernel against HBM bandwidth: the algorithmic bytes of one step,
 N*(8*(8d+1)+16) + 5*M (N nodes, M directed bond slots, d=3), over the
 step kernel's average duration, which libpg measures with HIP events
@@ -276,6 +278,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batched-steps", type=int, default=512)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form leg")
+    ap.add_argument("--tuning", default="", metavar="SKIN,BLOCK",
+                    help="Verlet skin factor and block size instead of the autotuner's choice")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 2D two-disk and bulk-settle workloads")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -302,7 +306,18 @@ def main():
     print(f"bench: workload built in {time.perf_counter() - t0:.1f} s: {setup.n_nodes} nodes, "
           f"{setup.n_slots()} directed bonds", file=sys.stderr, flush=True)
     dim = setup.dim
+    # Verlet skin and block size from the autotuner, run on a separate handle
+    # of the same workload so that the timed steps are the same ones
+    if args.tuning:
+        skin, block = float(args.tuning.split(",")[0]), int(args.tuning.split(",")[1])
+    else:
+        t0 = time.perf_counter()
+        with Stepper(setup, params, device=0) as tune:
+            skin, block = tune.autotune(trial_steps=128)
+        print(f"bench: autotuned skin factor {skin}, block {block} in "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     with Stepper(setup, params, device=0) as g:
+        g.set_tuning(skin, block)
         g.step(args.warmup)
         g.sync()
         st0 = g.stats()
@@ -366,6 +381,7 @@ def main():
             "dt": params.dt,
             "fracture": bool(params.enable_fracture),
             "top_wall_speed": float(params.wall_speed[5]),
+            "tuning": {"skin_factor": skin, "block": block},
         },
         "roofline": {
             "bound": "hbm",
