@@ -86,6 +86,18 @@ def test_two_disk_collision_bit_exact(oracle_threads):
     assert len(o.contacts()) > 0, "workload should be in contact by t=400"
 
 
+def test_reference_two_disk_case_full_run_bit_exact(oracle_threads):
+    # the reference's own small case (2 disks at +-10 m/s, dt 1e-8) over its
+    # whole 5000-step run
+    s, p = setups.two_disk_collision(seed=0)
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1000, 2500, 5000):
+        _advance(g, o, t - done)
+        done = t
+        _assert_exact(g, o, f"t={t}")
+
+
 def test_two_disk_fracture_bit_exact(oracle_threads):
     s, p = setups.two_disk_collision(seed=4, gap=1e-5, speed=50.0)
     # disk 0 expands radially: its bonds stretch past s0 within ~30 steps
