@@ -258,6 +258,34 @@ def test_self_contact_rules():
     assert F[1, 0] == pytest.approx(-(bond + contact), rel=1e-12)
 
 
+def test_self_contact_beyond_contact_radius_uses_rc():
+    """Same-grain pair with a broken bond whose reference distance rho0 is
+    at least Rc (contact.h:722-727): R = Rc and K = c/Rc, not rho0 and
+    c/rho0.  The pair is squeezed from rho0 = 1.5 Rc to r = 0.75 Rc in one
+    step, so the three candidate laws give three different forces."""
+    Rc, c, V = 2e-5, 3e19, 1e-12
+    d0 = 1.5 * Rc
+    X = np.array([[0.0, 0.0], [d0, 0.0]])
+    g = grain(X, vol=[V, V], conn=[(0, 1)], cnot=c, snot=-0.6)
+    s, p = build([g], 2, Rc=Rc, Kn=1.0, enable_fracture=1, self_contact=1, allow_damping=0,
+                 allow_friction=0)
+    s.disp[1, 0] = -0.5 * d0  # stretch -0.5 > s0 = -0.6: the bond breaks in step 1
+    o = oracle.Oracle(s, p)
+    o.step(1, record_contacts=True)
+    assert (o.bond_intact() == 0).all()
+    assert o.contacts().tolist() == [[0, 1], [1, 0]]
+    r = d0 - 0.5 * d0
+    assert r <= Rc < d0
+    bond = c * ((r - d0) / d0) * V  # on node 0 along +x (negative: pushes it away)
+    want = bond - (c / Rc) * (Rc - r) * V
+    F = o.read("force")
+    assert F[0, 0] == pytest.approx(want, rel=1e-12)
+    assert F[1, 0] == pytest.approx(-want, rel=1e-12)
+    # the plausible slips: K = c/rho0, or R = rho0
+    for wrong in (bond - (c / d0) * (Rc - r) * V, bond - (c / Rc) * (d0 - r) * V):
+        assert abs(F[0, 0] - wrong) > 1e-3 * abs(wrong)
+
+
 def test_bbox_pruned_pairs_equal_brute_force():
     """get_minmax + intersects_box_fext prune grain pairs without losing any
     node pair within Rc: the contact list equals an O(N^2) search."""
@@ -340,6 +368,96 @@ def test_wall_moving_relative_velocity_3d():
     assert F[0, 2] == pytest.approx(fn + fd, rel=1e-12)
     w, _ = o.wall()
     assert w[5] == 1.0 + (-3.0) * DT
+
+
+def _seg_area_2d(R, d):
+    """Circular segment of a disk of radius R cut at distance d from the
+    centre (the 2D 'mass' of the wall law), from its textbook closed form."""
+    return R * R * math.acos(d / R) - d * math.sqrt(R * R - d * d)
+
+
+@pytest.mark.parametrize("vt", [5.0, -5.0, 1e-6, 1e-11, -1e-11])
+def test_wall_friction_2d_sign_law_and_rho_over_dt_cap(vt):
+    """2D wall friction (contact.h:345-376) on a node sliding along the bottom
+    wall: -mu |F_w| sign(v_t) while |v_t| >= mu |F_w| dt / rho, else the cap
+    -v_t rho / dt (the 2D law multiplies by rho; the 3D one divides).  The
+    velocities straddle both forms of the threshold: 1e-6 slides under
+    mu|F_w| dt/rho but would be capped under mu|F_w| dt rho.  The reaction on
+    the bottom wall is -V times the friction density."""
+    Rc, K, V, rho, mu = 2e-5, 1e16, 2e-12, 1000.0, 0.3
+    d = 0.4 * Rc
+    g = grain(np.array([[0.0, d]]), vol=[V], rho=rho)
+    s, p = build([g], 2, Rc=Rc, Kn=K, mu=mu, vels=[np.array([vt, 0.0])],
+                 wall=[-1.0, 1.0, 1.0, 0.0], allow_damping=0, allow_friction=1)
+    F, o = first_force(s, p)
+    fw = K * Rc * _seg_area_2d(Rc, d)  # centroid 0 in 2D
+    thr = mu * fw * DT / rho
+    if abs(vt) < thr:
+        want = -vt * rho / DT
+        wrong = -vt / rho / DT
+    else:
+        want = -mu * fw * math.copysign(1.0, vt)
+        wrong = -vt * rho / DT if abs(vt) < mu * fw * DT * rho else -mu * fw * (-math.copysign(1.0, vt))
+    assert (abs(vt) < thr) == (abs(vt) < 1e-9)
+    assert F[0, 0] == pytest.approx(want, rel=1e-12)
+    assert abs(F[0, 0] - wrong) > 1e-3 * abs(want)
+    assert F[0, 1] == pytest.approx(fw, rel=1e-12)
+    _, react = o.wall()
+    assert react[3, 0] == pytest.approx(-V * want, rel=1e-12)
+    assert react[3, 1] == pytest.approx(-V * fw, rel=1e-12)
+
+
+def _cap_3d(R, d):
+    """The reference's 3D wall 'mass' and centroid (contact.h:30-36):
+    A = pi/3 (R-d)^2 (2R-d), I = pi/4 (R^2-d^2)^2, centroid I/A."""
+    A = math.pi / 3 * (R - d) ** 2 * (2 * R - d)
+    return (math.pi / 4 * (R * R - d * d) ** 2) / A, A
+
+
+@pytest.mark.parametrize("side,vel", [
+    (2, (3.0, -4.0, 0.0)),        # z_min wall: tangents x, y; sliding
+    (2, (3e-20, -4e-20, 0.0)),    # z_min wall; capped
+    (0, (0.0, 2.0, 1.0)),         # x_min wall: tangents y, z; sliding
+    (4, (1e-20, 0.0, 2e-20)),     # y_max wall: tangents z, x; capped
+    (5, (1e-12, 0.0, 0.0)),       # z_max wall; slides, but mu|F_w| dt rho would cap it
+])
+def test_wall_friction_3d_direction_law_and_over_rho_over_dt_cap(side, vel):
+    """3D wall friction (contact.h:377-409): with the tangential relative
+    velocity v_p (components (dir+1)%3, (dir+2)%3) the force is
+    -mu |F_w| v_p/|v_p| while |v_p| >= mu |F_w| dt/rho, else the cap
+    -(|v_p|/rho/dt) v_p/|v_p| = -v_p/(rho dt).  Sides on three axes pin the
+    tangent indices; the capped cases fail the 2D law's rho/dt."""
+    Rc, K, V, rho, mu = 2e-5, 1e16, 2e-12, 1000.0, 0.3
+    d = 0.2 * Rc
+    walls = [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]  # x_min, y_min, z_min, x_max, y_max, z_max
+    ax = side % 3
+    x0 = np.zeros(3)
+    x0[ax] = walls[side] + (d if side < 3 else -d)
+    g = grain(x0[None, :], vol=[V], rho=rho)
+    s, p = build([g], 3, Rc=Rc, Kn=K, mu=mu, vels=[np.array(vel)], wall=walls,
+                 allow_damping=0, allow_friction=1)
+    F, o = first_force(s, p)
+    x = o.read("CurrPos")[0]
+    dist = abs(x[ax] - walls[side])
+    c, A = _cap_3d(Rc, dist)
+    fw = K * max(Rc - c, 0.0) * A
+    assert fw > 0
+    t1, t2 = (ax + 1) % 3, (ax + 2) % 3
+    vp = np.array([vel[t1], vel[t2]])
+    vn = float(np.hypot(*vp))
+    capped = vn < mu * fw * DT / rho
+    assert capped == (vn < 1e-14)
+    want = -vp / (rho * DT) if capped else -mu * fw * vp / vn
+    got = np.array([F[0, t1], F[0, t2]])
+    assert got == pytest.approx(want, rel=1e-12)
+    if capped:  # the 2D law's -v rho/dt would be rho^2 times larger
+        assert np.all(np.abs(got - (-vp * rho / DT)) > 1e-3 * np.abs(-vp * rho / DT))
+    sgn = 1.0 if side < 3 else -1.0
+    assert F[0, ax] == pytest.approx(sgn * fw, rel=1e-12)
+    _, react = o.wall()
+    assert react[side, t1] == pytest.approx(-V * want[0], rel=1e-12)
+    assert react[side, t2] == pytest.approx(-V * want[1], rel=1e-12)
+    assert react[side, ax] == pytest.approx(-V * sgn * fw, rel=1e-12)
 
 
 # --------------------------------------------------------------- Verlet
@@ -520,6 +638,61 @@ def test_torque_is_tangential():
     F, _ = first_force(s, p)
     tang = np.stack([-np.sin(ang), np.cos(ang)], axis=1)
     assert np.allclose(F, 5.0 * tang, atol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_torque_3d_about_each_axis(axis):
+    """3D torque (timeloop.h:790-824) about torque_axis a: r = x - mean x
+    with its a-component dropped, force density tau (e_a x r)/|e_a x r|,
+    the right-handed tangent about e_a.  Nodes sit at several heights along
+    the axis, which the projection drops."""
+    rng = np.random.default_rng(11 + axis)
+    ang = np.linspace(0, 2 * np.pi, 7, endpoint=False) + 0.1
+    b, c = (axis + 1) % 3, (axis + 2) % 3
+    X = np.zeros((7, 3))
+    X[:, b] = 1e-4 * np.cos(ang)
+    X[:, c] = 1e-4 * np.sin(ang)
+    X[:, axis] = rng.uniform(-5e-5, 5e-5, size=7)
+    tau = 3.0
+    s, p = build([grain(X)], 3, enable_torque=1)
+    s.torque_axis[:] = axis
+    s.torque_val[:] = tau
+    F, o = first_force(s, p)
+    x = o.read("CurrPos")
+    r = x - x.mean(axis=0)
+    r[:, axis] = 0.0
+    e = np.zeros(3)
+    e[axis] = 1.0
+    tang = np.cross(e, r)
+    want = tau * tang / np.linalg.norm(tang, axis=1)[:, None]
+    assert np.allclose(F, want, rtol=0, atol=1e-9 * tau)
+    assert np.all(F[:, axis] == 0)
+    # a swapped pair of tangent indices gives the opposite sense of rotation
+    assert not np.allclose(F, -want, rtol=0, atol=1e-3 * tau)
+
+
+def test_set_stoppable_toggle():
+    """set_stoppable (timeloop.h:620-627): a grain that is not stoppable gets
+    no contact force from other grains (and no wall force) until step T,
+    when it becomes stoppable for good.  Grain 0 (movable, not stoppable)
+    touches grain 1 (not movable, so it never moves and computes nothing):
+    the force on grain 0 is zero at step T-1 and the closed-form repulsion
+    -K (Rc - r) V_1 at steps T and T+1."""
+    r, Rc, K, V1, T = 1.5e-5, 2e-5, 5e21, 3e-12, 6
+    s, p = _pair(2, r, Rc=Rc, Kn=K, Vj=V1, allow_damping=0, allow_friction=0,
+                 set_stoppable_index=0, set_stoppable_timestep=T)
+    s.stoppable[:] = [0, 1]
+    s.movable[:] = [1, 0]
+    o = oracle.Oracle(s, p)
+    o.step(T - 1)
+    assert np.all(o.read("force") == 0)
+    assert np.all(o.read("CurrPos") == s.pos), "nothing moved before step T"
+    o.step(1)
+    F = o.read("force")
+    assert F[0, 0] == pytest.approx(-K * (Rc - r) * V1, rel=1e-12)
+    assert np.all(F[1] == 0)
+    o.step(1)
+    assert o.read("force")[0, 0] < 0, "the grain stays stoppable after step T"
 
 
 # ------------------------------------------------------------ generators
