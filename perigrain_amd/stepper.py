@@ -4,12 +4,26 @@
 it owns the packed device state of one simulation (or a batch) and advances
 it step by step; snapshots are read back in the reference's layouts.
 """
+import atexit
 import ctypes
+import weakref
 
 import numpy as np
 
 from . import _ffi
 from ._ffi import check, lib
+
+# Handles still open when the interpreter exits are destroyed by this hook,
+# which runs before the interpreter tears down its modules and before the
+# HIP runtime's own exit handlers, rather than by __del__ during or after
+# that teardown.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_handles():
+    for st in list(_live):
+        st.close()
 
 
 class Stepper:
@@ -34,6 +48,7 @@ class Stepper:
         h = ctypes.c_void_p()
         check(L.pg_batch_create(sarr, parr, n, int(device), ctypes.byref(h)))
         self._h = h
+        _live.add(self)
         self.dim = int(setups[0].dim)
         self.n_instances = n
         self.setups = setups
@@ -42,8 +57,9 @@ class Stepper:
     # ------------------------------------------------------------- control
     def close(self):
         if getattr(self, "_h", None):
-            lib().pg_destroy(self._h)
-            self._h = None
+            h, self._h = self._h, None
+            _live.discard(self)
+            lib().pg_destroy(h)
 
     def __del__(self):
         try:

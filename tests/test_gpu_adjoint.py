@@ -113,3 +113,59 @@ def test_3d_adjoint_matches_cpu_dual_numbers(oracle_threads):
     raw_cpu = np.einsum("tsk,tskq->q", bar, dR)
     print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
     np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)
+
+
+def _fracturing_compression():
+    # the small 2D compression with fracture on; the grain under the top
+    # wall expands radially (2e4 1/s, 50% per-node spread), so its bonds
+    # break from step 5 on, inside the first and second checkpoint segments
+    from perigrain_amd import setups
+    s, p = setups.compression_2d_diff(n_grains=4, nodes_per_grain=60)
+    p.enable_fracture = 1
+    g = int(np.argmax([s.pos[s.grain_nodes(k), 1].max() for k in range(s.n_grains)]))
+    nd = s.grain_nodes(g)
+    r = s.pos[nd] - s.pos[nd].mean(axis=0)
+    rng = np.random.default_rng(5)
+    s.vel[nd] += 2e4 * r * (1.0 + 0.5 * rng.standard_normal((r.shape[0], 1)))
+    return s, p
+
+
+def test_adjoint_with_bonds_breaking_mid_segment(oracle_threads):
+    # Bond status frozen to the forward pass, with real breaks: the step of
+    # each break is recorded, each segment's recompute restores the bonds
+    # broken before it, and the adjoint of step t uses a bond only while it
+    # was intact at t (the break happens after the step's force used it).
+    # The gradient is checked against the CPU dual-number oracle, which
+    # breaks the same bonds on their values, and the state after the
+    # backward pass is the state after the forward pass.
+    import oracle
+    from perigrain_amd import Stepper
+    s, p = _fracturing_compression()
+    steps, ck = 120, 30
+    with Stepper(s, p, device=0) as probe:  # where the breaks happen
+        broken = []
+        for t in range(1, steps + 1):
+            probe.step(1)
+            broken.append(int((probe.bond_intact() == 0).sum()))
+    new = [t for t in range(1, steps + 1) if broken[t - 1] > (broken[t - 2] if t > 1 else 0)]
+    print(f"bonds break at steps {new[:12]}..., {broken[-1]} broken slots at t={steps}", flush=True)
+    assert any(t % ck not in (0, 1) for t in new), "a break strictly inside a segment"
+    assert len(new) >= 5 and broken[-1] < s.n_slots()
+    R, dR = oracle.reaction_history_tangent(s, p, steps, threads=oracle_threads)
+    with Stepper(s, p, device=0) as g:
+        hist = g.adjoint_forward(steps, ck)
+        np.testing.assert_allclose(hist, R, rtol=1e-10, atol=1e-10 * np.abs(R).max())
+        assert np.any(hist[:, 2, 1] != 0), "the top wall should be in contact"
+        x0, v0, b0 = g.read("CurrPos"), g.read("vel"), g.bond_intact()
+        assert int((b0 == 0).sum()) == broken[-1]
+        bar = np.zeros_like(hist)
+        bar[:, 2] = 2.0 * hist[:, 2]  # misfit against a zero target
+        raw = g.adjoint_backward(bar)
+        assert g.t == steps
+        assert np.array_equal(g.read("CurrPos"), x0) and np.array_equal(g.read("vel"), v0)
+        assert np.array_equal(g.bond_intact(), b0), "the broken-bond set after the backward pass"
+    bar_c = np.zeros_like(R)
+    bar_c[:, 2] = 2.0 * R[:, 2]
+    raw_cpu = np.einsum("tsk,tskq->q", bar_c, dR)
+    print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
+    np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)

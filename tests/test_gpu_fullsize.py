@@ -119,14 +119,55 @@ def test_compression_3d_full_size(oracle_threads):
         assert g.t == 200
 
 
-def test_batched_256_instances_equal_single_runs():
+def _assert_instance(b, k, o, what):
+    """Instance k of a handle against an oracle run of that instance: bond
+    status, contact pairs and wall positions bit for bit; the fields and the
+    wall reaction bit for bit unless a node touched a 2D wall (asin, whose
+    last bit the host and device libraries may round differently), then to
+    1e-10 of each field's largest value."""
+    walls = bool(np.any(o.wall()[1] != 0))
+    for f in FIELDS:
+        a, c = b.read(f, inst=k), o.read(f)
+        if walls:
+            err = np.max(np.abs(a - c)) / max(np.max(np.abs(c)), 1e-300)
+            assert err <= 1e-10, f"{what} {f}: {err:.3e}"
+        else:
+            assert np.array_equal(a, c), f"{what} {f}: {int((a != c).sum())} entries differ"
+    assert np.array_equal(b.bond_intact(inst=k), o.bond_intact()), f"{what} bond status"
+    assert np.array_equal(b.contacts(inst=k), o.contacts()), f"{what} contact pairs"
+    wb, rb = b.wall(inst=k)
+    wo, ro = o.wall()
+    assert np.array_equal(wb, wo), f"{what} wall positions"
+    if walls:
+        assert np.max(np.abs(rb - ro)) <= 1e-10 * np.max(np.abs(ro)), f"{what} wall reaction"
+    else:
+        assert np.array_equal(rb, ro), f"{what} wall reaction"
+    return walls
+
+
+def test_batched_256_instances_equal_single_runs(oracle_threads):
     S, P = setups.batched_settle(256, seed=0)
-    print(f"batched: {len(S)} instances, {sum(x.n_nodes for x in S)} nodes", flush=True)
+    sizes = [x.n_nodes for x in S]
+    small, large = int(np.argmin(sizes)), int(np.argmax(sizes))
+    print(f"batched: {len(S)} instances, {sum(sizes)} nodes; smallest {small} "
+          f"({sizes[small]} nodes), largest {large} ({sizes[large]} nodes)", flush=True)
     with Stepper(S, P, device=0) as b:
         b.set_graph(True)
         b.autotune(trial_steps=8)  # advances the run by 9 * 8 steps
         b.step(160 - b.t)
         assert b.t == 160
+        # instances of the batched handle against the oracle's run of each
+        # (timeloop.h:608-878 per instance), bit for bit
+        for k in sorted({small, large, 0, 97, 255}):
+            o = oracle.Oracle(S[k], P[k], threads=oracle_threads)
+            try:
+                o.step(160, record_contacts=True)
+                walls = _assert_instance(b, k, o, f"instance {k} vs oracle")
+                print(f"instance {k}: {sizes[k]} nodes, {len(o.contacts())} contact pairs, "
+                      f"{'wall contact' if walls else 'no wall contact'}: matches the oracle",
+                      flush=True)
+            finally:
+                o.close()
         for k in (0, 1, 97, 200, 255):
             with Stepper(S[k], P[k], device=0) as one:
                 one.step(160)
@@ -137,6 +178,52 @@ def test_batched_256_instances_equal_single_runs():
                 wb, rb = b.wall(inst=k)
                 w1, r1 = one.wall()
                 assert np.array_equal(wb, w1) and np.array_equal(rb, r1), f"instance {k} walls"
+
+
+def _settle_2d_against_oracle(s, p, checkpoints, threads, what):
+    print(f"{what}: {s.n_nodes} nodes, {s.n_grains} grains, {s.n_slots()} directed bonds",
+          flush=True)
+    from test_gpu_parity import _assert_close
+    with Stepper(s, p, device=0) as g:
+        o = oracle.Oracle(s, p, threads=threads)
+        try:
+            done = 0
+            for t in checkpoints:
+                g.step(t - done)
+                o.step(t - done, record_contacts=True)
+                done = t
+                exact = all(np.array_equal(g.read(f), o.read(f)) for f in FIELDS)
+                print(f"{what} t={t}: fields {'bit-identical' if exact else 'within tolerance'}, "
+                      f"{len(o.contacts())} contact pairs, "
+                      f"{int((o.bond_intact() == 0).sum())} broken slots", flush=True)
+                _assert_close(g, o, 1e-10, f"{what} t={t}")
+            return o.contacts(), o.bond_intact(), o.wall()[1]
+        finally:
+            o.close()
+
+
+def test_bulk_settle_2d_full_size(oracle_threads):
+    # the 2D bulk settle at its own size and settings: 400 grains (disks,
+    # squares, plus shapes, annuli), 121,900 nodes, gravity, 4 walls,
+    # fracture on, dt 2e-8, 1000 steps against the oracle
+    s, p = setups.bulk_settle_2d(seed=0)
+    assert s.n_grains == 400 and p.enable_fracture == 1 and s.allow_wall == 1
+    _settle_2d_against_oracle(s, p, (1, 500, 1000), oracle_threads, "bulk settle")
+
+
+def test_bulk_settle_2d_full_size_colliding(oracle_threads):
+    # the same workload with every grain thrown at 20 m/s in a random
+    # direction, so that grains collide with each other and with the walls
+    # within the horizon
+    s, p = setups.bulk_settle_2d(seed=0)
+    rng = np.random.default_rng(1)
+    for gi in range(s.n_grains):
+        a = rng.uniform(0, 2 * np.pi)
+        s.vel[s.grain_nodes(gi)] = 20.0 * np.array([np.cos(a), np.sin(a)])
+    pairs, _, react = _settle_2d_against_oracle(s, p, (1, 300, 600), oracle_threads,
+                                                "bulk settle, colliding")
+    assert len(pairs) > 0, "grains should collide"
+    assert np.any(react != 0), "grains should reach the walls"
 
 
 def test_adjoint_2000_steps(oracle_threads):

@@ -63,6 +63,26 @@ def _torque(s, p):
     s.torque_val = np.array([2e6, -1e6] + [5e5] * (s.n_grains - 2))
 
 
+def _torque_axes_x_y(s, p):
+    # 3D torque about x and about y (timeloop.h:790-824), alternating by grain
+    p.enable_torque = 1
+    s.torque_axis = np.array([k % 2 for k in range(s.n_grains)], dtype=np.uint32)
+    s.torque_val = np.array([3e6, -2e6] + [1e6] * (s.n_grains - 2))
+
+
+def _bdry_set_order(s, p):
+    # nonlocal boundary lists in an arbitrary (not ascending) order, as
+    # exp_dict.py:300 writes them from a Python set: the contact order
+    # follows the list (contact.h:672-681)
+    p.nl_bdry_only = 1
+    rng = np.random.default_rng(13)
+    parts = []
+    for g in range(s.n_grains):
+        b = s.bdry_nodes[s.bdry_offset[g]:s.bdry_offset[g + 1]].copy()
+        parts.append(b[rng.permutation(len(b))])
+    s.bdry_nodes = np.concatenate(parts).astype(np.uint32)
+
+
 def _clamped(s, p):
     # five nodes of the last grain are held at zero displacement
     s.clamped_offset = np.zeros(s.n_grains + 1, dtype=np.int64)
@@ -95,6 +115,9 @@ CASES = {
     "3d-nl_bdry_only-nonuniform-volume": (_compression_3d, [_nl_bdry_only, _nonuniform_volume]),
     "3d-torque-clamped": (_compression_3d, [_torque, _clamped]),
     "3d-no-damping-no-friction": (_compression_3d, [_no_damping_no_friction]),
+    "3d-torque-axes-x-y": (_compression_3d, [_torque_axes_x_y]),
+    "2d-nl_bdry_only-set-order": (_collision, [_bdry_set_order]),
+    "3d-nl_bdry_only-set-order": (_compression_3d, [_bdry_set_order]),
 }
 
 
