@@ -143,6 +143,16 @@ int pg_device_count(int *n);
 int pg_check_arith(int device, uint64_t n, const double *a, const double *b, double *q,
                    double *r);
 
+/* Diagnostics (no reference counterpart): the bond loop's own arithmetic on
+ * given vectors xi and eta ([n][dim]): s[i] = (|eta| - |xi|)/|xi| and
+ * e[i] = eta/|eta| as the step kernel computes them (particle2.h:134-146).
+ * check = 1: with its per-bond range test and IEEE fallback (any input);
+ * check = 0: the shortcut the kernel takes when every coordinate of the
+ * step is "nice" (zero or of magnitude in [2^-240, 2^240]), valid only for
+ * such inputs. */
+int pg_check_bond(int device, int dim, int check, uint64_t n, const double *xi, const double *eta,
+                  double *s, double *e);
+
 /* Create a stepper for one simulation on HIP device `device`.
  * Replaces the state set-up of run_timeloop (timeloop.h:504-557; gen_xi at
  * particle2.h:236-252 and conn2NArr at load_hdf5.h:28-38) without the file
@@ -223,6 +233,7 @@ int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs,
 #define PG_GRAIN_STATS(dim) (5 * (dim) + 1)
 int pg_grain_stats(pg_handle *h, int inst, double *out);
 
+
 /* Stream (hipStream_t) the handle enqueues its kernels on. */
 int pg_stream(pg_handle *h, void **stream);
 
@@ -282,6 +293,28 @@ int pg_event_elapsed(pg_handle *h, int a, int b, float *ms);
  * those runs, so *ms / *launches is the average step-kernel duration
  * (including any per-step grain-mean launches when torque is on). */
 int pg_kernel_time(pg_handle *h, double *ms, uint64_t *launches);
+
+/* Snapshots that overlap with stepping.  The reference writes a snapshot
+ * between steps (timeloop.h:881-951); these calls let a caller write it from
+ * another thread while the steps go on.  pg_snapshot_take enqueues on the
+ * handle's stream, after the steps enqueued so far, a device copy of what a
+ * snapshot of the last completed step holds -- CurrPos, vel, acc, force,
+ * the bond status, the wall positions and reaction -- into snapshot slot
+ * `slot` (0 .. PG_SNAPSHOT_SLOTS-1, device memory the handle allocates on the
+ * slot's first use) and returns at once.  The pg_snapshot_* readers give the
+ * same values as pg_read_nodes / pg_read_conn / pg_read_wall would have given
+ * right after that step: they wait for the slot's copy, then copy to host
+ * memory on a stream of their own, so one other host thread may call them
+ * while the handle's own thread keeps calling pg_step.  A slot must not be
+ * taken again until its reader is done with it. */
+#define PG_SNAPSHOT_SLOTS 2
+int pg_snapshot_take(pg_handle *h, int slot);
+int pg_snapshot_step(pg_handle *h, int slot, uint32_t *t);
+int pg_snapshot_read_nodes(pg_handle *h, int slot, int inst, int field, double *dst);
+int pg_snapshot_read_conn(pg_handle *h, int slot, int inst, int grain, uint32_t *dst,
+                          uint64_t cap_rows, uint64_t *rows);
+int pg_snapshot_read_wall(pg_handle *h, int slot, int inst, double *wall_info,
+                          double *reaction);
 
 #ifdef __cplusplus
 }

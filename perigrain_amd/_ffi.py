@@ -197,10 +197,18 @@ def lib():
         "pg_event_record": [H, ctypes.c_int],
         "pg_event_elapsed": [H, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
         "pg_kernel_time": [H, _dp, _u64p],
+        "pg_snapshot_take": [H, ctypes.c_int],
+        "pg_snapshot_step": [H, ctypes.c_int, _u32p],
+        "pg_snapshot_read_nodes": [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
+        "pg_snapshot_read_conn": [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p,
+                                  ctypes.c_uint64, _u64p],
+        "pg_snapshot_read_wall": [H, ctypes.c_int, ctypes.c_int, _dp, _dp],
         "pg_adjoint_forward": [H, ctypes.c_uint32, ctypes.c_uint32, _dp],
         "pg_adjoint_backward": [H, _dp, _dp],
         "pg_device_count": [ctypes.POINTER(ctypes.c_int)],
         "pg_check_arith": [ctypes.c_int, ctypes.c_uint64, _dp, _dp, _dp, _dp],
+        "pg_check_bond": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _dp, _dp,
+                          _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

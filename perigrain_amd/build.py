@@ -87,7 +87,7 @@ def build_io():
     outs = [out_io]
     for dim in (2, 3):
         exe = os.path.join(BIN, f"simulate{dim}d")
-        _run(cxx + [f"-DPG_DIM={dim}", os.path.join(HOST, "simulate.cpp"), "-o", exe + ".tmp",
+        _run(cxx + ["-pthread", f"-DPG_DIM={dim}", os.path.join(HOST, "simulate.cpp"), "-o", exe + ".tmp",
                     "-L", HERE, "-lpg", "-lpgio", "-L", hdir, "-l:" + soname,
                     "-Wl,-rpath,$ORIGIN/..:$ORIGIN/hdf5"])
         os.replace(exe + ".tmp", exe)

@@ -117,8 +117,17 @@ struct pg_handle {
   unsigned long long *bbox[3] = {nullptr, nullptr, nullptr};
   double *wallpos[2] = {nullptr, nullptr};
   uint32_t *violation = nullptr;
+  // niceness flags of u by step (StepArgs::unice) and whether X and the bond
+  // lengths allow the unchecked bond loop (StepArgs::xi_ok)
+  uint32_t *unice = nullptr;
+  uint32_t unice_host[3] = {1, 1, 0};
+  bool xi_ok = false;
   double *force_out = nullptr, *wterm = nullptr, *reaction = nullptr;
   uint8_t *wflag = nullptr;
+  // wall-reaction scratch: compacted hit keys and per-chunk counts, one set
+  // for the plain recording step (rhits, rcnt) and one per ring slot of a
+  // recorded run (rhits_ring, rcnt_ring)
+  uint32_t *rhits = nullptr, *rcnt = nullptr, *rhits_ring = nullptr, *rcnt_ring = nullptr;
   // list build scratch
   GridDev *grids = nullptr;
   unsigned long long *ibox = nullptr;
@@ -175,10 +184,28 @@ struct pg_handle {
   // host copy of the bond table, valid for step bcol_host_t
   std::vector<uint32_t> bcol_host;
   uint32_t bcol_host_t = 0xFFFFFFFFu;
+  // snapshot slots (pg_snapshot_*): device copies of a step's snapshot
+  // state, read back on rstream by another host thread
+  struct Snap {
+    bool alloc = false, taken = false;
+    uint32_t t = 0;
+    double *rec = nullptr, *vel = nullptr, *acc = nullptr, *force = nullptr;
+    double *wall = nullptr, *reaction = nullptr;
+    uint32_t *bcol = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<int32_t> yshift_grain; // per instance, as at the take
+    std::vector<double> yshift;
+    std::vector<uint32_t> bcol_host; // filled by the first Connectivity read
+    bool bcol_ready = false;
+  } snap[PG_SNAPSHOT_SLOTS];
+  hipStream_t rstream = nullptr;
   StepArgs A{};
 };
 
 namespace {
+
+// chunks of the wall-reaction compaction for ck contact keys
+size_t react_chunks(size_t ck) { return (ck + kWflagSpare - 1) / kWflagSpare; }
 
 template <class T> int dalloc(pg_handle *h, T *&p, size_t n) {
   if (n == 0) n = 1;
@@ -207,11 +234,14 @@ template <class T> int h2d(pg_handle *h, T *dst, const T *src, size_t n) {
   HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
   return PG_OK;
 }
-template <class T> int d2h(pg_handle *h, T *dst, const T *src, size_t n) {
+template <class T> int d2h_on(hipStream_t st, T *dst, const T *src, size_t n) {
   if (n == 0) return PG_OK;
-  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return PG_OK;
+}
+template <class T> int d2h(pg_handle *h, T *dst, const T *src, size_t n) {
+  return d2h_on(h->stream, dst, src, n);
 }
 
 #define RC(x)                  \
@@ -253,6 +283,8 @@ void fill_args(pg_handle *h) {
   A.wallpos[1] = h->wallpos[1];
   A.gmean = h->gmean;
   A.violation = h->violation;
+  A.unice = h->unice;
+  A.xi_ok = h->xi_ok ? 1 : 0;
   A.force_out = h->force_out;
   A.wterm = h->wterm;
   A.wflag = h->wflag;
@@ -406,6 +438,19 @@ int bond_status(pg_handle *h, const std::vector<uint32_t> *&st) {
     h->bcol_host_t = h->t;
   }
   st = &h->bcol_host;
+  return PG_OK;
+}
+
+// Niceness flags for a run that continues with step t+1 from state that
+// no step kernel checked: u^{t+1} unknown (per-bond tests), the flag of
+// u^{t+2} cleared for step t+1 to set.
+int set_unice(pg_handle *h, uint32_t t) {
+  h->unice_host[(t + 1) % 3] = 1;
+  h->unice_host[(t + 2) % 3] = 0;
+  h->unice_host[t % 3] = 1;
+  HIPCHK(hipMemcpyAsync(h->unice, h->unice_host, sizeof(h->unice_host), hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream)); // the host array is reused
   return PG_OK;
 }
 
@@ -675,6 +720,25 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
         return fail(PG_ERR_INVALID, "Connectivity lists a bond twice (node " + std::to_string(i) + ")");
     }
   }
+  {
+    // every coordinate of X nice (zero or 2^-240 <= |x| <= 2^240, see
+    // nice() in pg_device.h) and no bond of length zero: then the bond loop
+    // needs no per-bond range test while u is nice too
+    auto nice_h = [](double x) {
+      const double m = std::fabs(x);
+      return m == 0.0 || (m >= 0x1p-240 && m <= 0x1p240);
+    };
+    bool ok = true;
+    for (uint64_t i = 0; i < N && ok; ++i)
+      for (int k = 0; k < D; ++k) ok = ok && nice_h(rec[i].X[k]);
+    for (uint64_t i = 0; i < N && ok; ++i)
+      for (uint64_t r = row_ptr[i]; r < row_ptr[i + 1] && ok; ++r) {
+        bool same = true;
+        for (int k = 0; k < D; ++k) same = same && rec[rows[r]].X[k] == rec[i].X[k];
+        ok = !same;
+      }
+    h->xi_ok = ok;
+  }
   for (int s = 0; s < n; ++s) {
     h->inst[s].slot_begin = row_ptr[h->inst[s].node_begin];
     h->inst[s].slot_end = row_ptr[h->inst[s].node_end];
@@ -845,9 +909,12 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   RC(dalloc(h, h->wallpos[1], (size_t)n * 6));
   RC(dalloc(h, h->gmean, h->any_torque ? (size_t)G * 3 : 1));
   RC(dalloc(h, h->violation, 1));
+  RC(dalloc(h, h->unice, 3));
   RC(dalloc(h, h->tbase, 1));
   RC(dalloc(h, h->wterm, (size_t)(CK ? CK : 1) * 24));
-  RC(dalloc(h, h->wflag, (size_t)CK + kWflagSpare)); // zero spare: k_reaction reads whole chunks
+  RC(dalloc(h, h->wflag, (size_t)CK + kWflagSpare)); // zero spare: the reaction reads whole chunks
+  RC(dalloc(h, h->rhits, react_chunks(CK) * kWflagSpare));
+  RC(dalloc(h, h->rcnt, react_chunks(CK)));
   RC(dalloc(h, h->reaction, (size_t)n * 18));
   RC(dalloc(h, h->grids, n));
   RC(dalloc(h, h->ibox, (size_t)n * 2 * D));
@@ -913,6 +980,7 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   HIPCHK(hipMemsetAsync(h->gmean, 0, (h->any_torque ? (size_t)G * 3 : 1) * sizeof(double), h->stream));
   for (int k = 0; k < 3; ++k) HIPCHK(launch_fill_box(h->bbox[k], (size_t)G * 2 * D, D, h->stream));
   HIPCHK(hipMemsetAsync(h->violation, 0xFF, sizeof(uint32_t), h->stream));
+  RC(set_unice(h, 0));
   fill_args(h);
 
   // ---- position update of step 1 and the first neighbour list
@@ -923,19 +991,36 @@ int create_impl(const pg_setup *setups, const pg_params *params, int n, int devi
   return PG_OK;
 }
 
+// Teardown in reverse order of dependence: wait for the stream's work; drop
+// the captured graph, whose kernel nodes hold pointers into the handle's
+// buffers, while those buffers still exist; destroy the stream before the
+// events recorded on it and the pinned buffer its copies used; free the
+// device memory last.
 void destroy_impl(pg_handle *h) {
   if (!h) return;
+  (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (void *p : h->allocs) (void)hipFree(p);
-  h->allocs.clear();
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
-  if (h->h_pin) (void)hipHostFree(h->h_pin);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
+  if (h->rstream) {
+    (void)hipStreamSynchronize(h->rstream);
+    (void)hipStreamDestroy(h->rstream);
+    h->rstream = nullptr;
+  }
+  for (auto &sn : h->snap)
+    if (sn.ev) (void)hipEventDestroy(sn.ev);
   for (int k = 0; k < 8; ++k)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
   for (int k = 0; k < 2; ++k)
     if (h->kev[k]) (void)hipEventDestroy(h->kev[k]);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->h_pin) (void)hipHostFree(h->h_pin);
+  h->h_pin = nullptr;
+  for (void *p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
   delete h;
 }
 
@@ -969,7 +1054,32 @@ size_t rec_len(const pg_handle *h) { return (size_t)h->ns * 2 * h->dim; }
 size_t vec_len(const pg_handle *h) { return (size_t)h->ns * h->dim; }
 size_t box_len(const pg_handle *h) { return (size_t)h->G * 2 * h->dim; }
 size_t wring_stride(const pg_handle *h) { return (size_t)(h->CK ? h->CK : 1) * 24; }
-size_t fring_stride(const pg_handle *h) { return (size_t)h->CK + kWflagSpare; }
+size_t fring_stride(const pg_handle *h) { return ((size_t)h->CK + kWflagSpare + 15) / 16 * 16; }
+
+// Wall reaction of the steps whose terms and flags lie in ring slots
+// first .. first + nsteps - 1 (modulo slots), into out (rstride per step).
+int reactions(pg_handle *h, const double *wterm, const uint8_t *wflag, uint32_t *hits,
+              uint32_t *cnt, uint32_t nsteps, uint32_t first, uint32_t slots, double *out,
+              size_t rstride) {
+  ReactionArgs R{};
+  R.insts = h->insts;
+  R.n_insts = h->I;
+  R.nchunks = (uint32_t)react_chunks(h->CK);
+  R.wterm = wterm;
+  R.wflag = wflag;
+  R.reaction = out;
+  R.hits = hits;
+  R.chunk_cnt = cnt;
+  R.first = first;
+  R.slots = slots;
+  R.wstride = wring_stride(h);
+  R.fstride = fring_stride(h);
+  R.rstride = rstride;
+  R.hstride = (size_t)R.nchunks * kWflagSpare;
+  R.cstride = R.nchunks;
+  HIPCHK(launch_reactions(h->dim, R, nsteps, h->stream));
+  return PG_OK;
+}
 
 // reset_partzero_y of instance s before step tr (timeloop.h:629-647): the
 // largest y among the other grains' nodes of pos + disp of step tr-1
@@ -1066,10 +1176,9 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     const uint32_t v = h->h_pin[0];
     if (h->hist) { // wall reactions of the steps that ran, in one launch
       const uint32_t ran = v > h->t ? std::min<uint32_t>(v - 1, h->t + chunk) - h->t : 0;
-      HIPCHK(launch_reactions(h->dim, h->I, h->insts, h->wring, h->fring,
-                              h->hist + (size_t)(h->t - h->hist_t0) * h->I * 18, ran,
-                              h->t % h->check_every, h->check_every, wring_stride(h),
-                              fring_stride(h), (size_t)h->I * 18, h->stream));
+      RC(reactions(h, h->wring, h->fring, h->rhits_ring, h->rcnt_ring, ran,
+                   h->t % h->check_every, h->check_every,
+                   h->hist + (size_t)(h->t - h->hist_t0) * h->I * 18, (size_t)h->I * 18));
     }
     if (v > h->t + chunk) { // no launch of this run returned early
       float ms = 0.f;
@@ -1091,7 +1200,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
                           (size_t)h->I * 18 * sizeof(double), hipMemcpyDeviceToDevice,
                           h->stream));
   else
-    HIPCHK(launch_reaction(h->dim, h->I, h->insts, h->wterm, h->wflag, h->reaction, h->stream));
+    RC(reactions(h, h->wterm, h->wflag, h->rhits, h->rcnt, 1, 0, 1, h->reaction, (size_t)h->I * 18));
   return PG_OK;
 }
 
@@ -1127,6 +1236,7 @@ int ck_restore(pg_handle *h, uint32_t k, uint32_t t) {
   RC(d2d(h, h->bbox[(t + 1) % 3], a.ck_box + k * box_len(h), box_len(h)));
   HIPCHK(launch_fill_box(h->bbox[(t + 2) % 3], box_len(h), h->dim, h->stream));
   HIPCHK(launch_adj_restore_bcol(h->d_bcol, a.break_step, h->E, t, h->stream));
+  RC(set_unice(h, t));
   h->t = t;
   h->bcol_host_t = 0xFFFFFFFFu;
   return build_list(h, t + 1);
@@ -1261,6 +1371,86 @@ int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
   return PG_OK;
 }
 
+// Node field `field` of instance inst in the reference's order and layout
+// ([n][dim]) from the given device arrays (records holding u of the step,
+// velocity, acceleration, force density), copied on stream st.  ysg/ysh: the
+// grain moved by reset_partzero_y and its y shift (-1: none).
+int read_nodes_from(pg_handle *h, int inst, int field, const double *rec, const double *vel,
+                    const double *acc, const double *force, int32_t ysg, double ysh,
+                    hipStream_t st, double *dst) {
+  const int D = h->dim;
+  const InstHost &ih = h->inst[inst];
+  const size_t n = ih.node_end - ih.node_begin;
+  // device order -> reference order (the permutation stays inside the range)
+  auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
+  if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
+    // record parts [D][ns][2] of the instance's node range; CurrPos = pos + disp
+    std::vector<double> r((size_t)D * n * 2);
+    for (int q = 0; q < D; ++q)
+      RC(d2h_on(st, r.data() + (size_t)q * n * 2, rec + rec_index(h->ns, ih.node_begin, 2 * q),
+                n * 2));
+    auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
+    // pos of the grain moved by reset_partzero_y carries the y shift
+    size_t s0 = 0, s1 = 0;
+    if (ysg >= 0) {
+      s0 = h->grain_node_begin[ysg] - ih.node_begin;
+      s1 = h->grain_node_begin[ysg + 1] - ih.node_begin;
+    }
+    for (size_t e = 0; e < n; ++e) {
+      const size_t a = at(e);
+      for (int k = 0; k < D; ++k) {
+        double X = el(a, k);
+        if (k == 1 && e >= s0 && e < s1) X = X + ysh;
+        dst[e * D + k] = field == PG_FIELD_CURRPOS ? X + el(a, D + k) : el(a, D + k);
+      }
+    }
+    return PG_OK;
+  }
+  const double *src = nullptr;
+  switch (field) {
+    case PG_FIELD_VEL: src = vel; break;
+    case PG_FIELD_ACC: src = acc; break;
+    case PG_FIELD_FORCE: src = force; break;
+    default: return fail(PG_ERR_INVALID, "unknown field");
+  }
+  std::vector<double> v(n);
+  for (int k = 0; k < D; ++k) {
+    RC(d2h_on(st, v.data(), src + (size_t)k * h->ns + ih.node_begin, n));
+    for (size_t e = 0; e < n; ++e) dst[e * D + k] = v[at(e)];
+  }
+  return PG_OK;
+}
+
+// Intact bonds of grain `grain` of instance inst as NbdArr2conn writes them,
+// from a host copy of the bond table.
+int read_conn_from(pg_handle *h, int inst, int grain, const std::vector<uint32_t> &st,
+                   uint32_t *dst, uint64_t cap_rows, uint64_t *rows) {
+  const InstHost &ih = h->inst[inst];
+  if (grain < 0 || grain >= (int)(ih.grain_end - ih.grain_begin))
+    return fail(PG_ERR_INVALID, "bad grain");
+  const uint32_t gg = ih.grain_begin + grain;
+  const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
+  uint64_t r = 0;
+  for (uint32_t ge = a; ge < b; ++ge) { // reference node order
+    const uint32_t i = h->inv[ge];
+    const uint32_t sl = i / SLICE, lane = i % SLICE;
+    for (uint32_t k = 0; k < h->deg[ge]; ++k) {
+      const uint64_t e = bslot(h->slice_base[sl], lane, k);
+      if (st[e] & BCOL_BROKEN) continue;
+      const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
+      if (li < q) {
+        if (dst && r < cap_rows) {
+          dst[2 * r] = li;
+          dst[2 * r + 1] = q;
+        }
+        ++r;
+      }
+    }
+  }
+  if (rows) *rows = r;
+  return PG_OK;
+}
+
 } // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -1352,54 +1542,10 @@ int pg_sizes(pg_handle *h, int inst, uint64_t *n_nodes, uint64_t *n_grains, uint
 int pg_read_nodes(pg_handle *h, int inst, int field, double *dst) {
   return guarded([&]() -> int {
     if (!h || !dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-    const int D = h->dim;
-    const InstHost &ih = h->inst[inst];
-    const size_t n = ih.node_end - ih.node_begin;
     HIPCHK(hipSetDevice(h->device));
-    // device order -> reference order (the permutation stays inside the range)
-    auto at = [&](size_t e) -> size_t { return h->inv[ih.node_begin + e] - ih.node_begin; };
-    // component k of the instance's node range of a [D][ns] array
-    auto comp = [&](const double *base, int k, std::vector<double> &v) -> int {
-      v.resize(n);
-      return d2h(h, v.data(), base + (size_t)k * h->ns + ih.node_begin, n);
-    };
-    if (field == PG_FIELD_CURRPOS || field == PG_FIELD_DISP) {
-      // record parts [D][ns][2] of the instance's node range; CurrPos = pos + disp
-      std::vector<double> r((size_t)D * n * 2);
-      for (int q = 0; q < D; ++q)
-        RC(d2h(h, r.data() + (size_t)q * n * 2,
-               h->rec[h->t & 1] + rec_index(h->ns, ih.node_begin, 2 * q), n * 2));
-      auto el = [&](size_t e, int m) { return r[((size_t)(m >> 1) * n + e) * 2 + (m & 1)]; };
-      // pos of the grain moved by reset_partzero_y carries the y shift
-      const InstDev &id = h->inst_dev[inst];
-      size_t s0 = 0, s1 = 0;
-      if (id.yshift_grain >= 0) {
-        s0 = h->grain_node_begin[id.yshift_grain] - ih.node_begin;
-        s1 = h->grain_node_begin[id.yshift_grain + 1] - ih.node_begin;
-      }
-      for (size_t e = 0; e < n; ++e) {
-        const size_t a = at(e);
-        for (int k = 0; k < D; ++k) {
-          double X = el(a, k);
-          if (k == 1 && e >= s0 && e < s1) X = X + id.yshift;
-          dst[e * D + k] = field == PG_FIELD_CURRPOS ? X + el(a, D + k) : el(a, D + k);
-        }
-      }
-      return PG_OK;
-    }
-    const double *src = nullptr;
-    switch (field) {
-      case PG_FIELD_VEL: src = h->vel[h->t & 1]; break;
-      case PG_FIELD_ACC: src = h->acc; break;
-      case PG_FIELD_FORCE: src = h->force_out; break;
-      default: return fail(PG_ERR_INVALID, "unknown field");
-    }
-    std::vector<double> v;
-    for (int k = 0; k < D; ++k) {
-      RC(comp(src, k, v));
-      for (size_t e = 0; e < n; ++e) dst[e * D + k] = v[at(e)];
-    }
-    return PG_OK;
+    const InstDev &id = h->inst_dev[inst];
+    return read_nodes_from(h, inst, field, h->rec[h->t & 1], h->vel[h->t & 1], h->acc,
+                           h->force_out, id.yshift_grain, id.yshift, h->stream, dst);
   });
 }
 
@@ -1459,33 +1605,9 @@ int pg_read_conn(pg_handle *h, int inst, int grain, uint32_t *dst, uint64_t cap_
                  uint64_t *rows) {
   return guarded([&]() -> int {
     if (!h || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
-    const InstHost &ih = h->inst[inst];
-    if (grain < 0 || grain >= (int)(ih.grain_end - ih.grain_begin))
-      return fail(PG_ERR_INVALID, "bad grain");
-    const uint32_t gg = ih.grain_begin + grain;
-    const uint32_t a = h->grain_node_begin[gg], b = h->grain_node_begin[gg + 1];
     const std::vector<uint32_t> *stp = nullptr;
     RC(bond_status(h, stp));
-    const std::vector<uint32_t> &st = *stp;
-    uint64_t r = 0;
-    for (uint32_t ge = a; ge < b; ++ge) { // reference node order
-      const uint32_t i = h->inv[ge];
-      const uint32_t sl = i / SLICE, lane = i % SLICE;
-      for (uint32_t k = 0; k < h->deg[ge]; ++k) {
-        const uint64_t e = bslot(h->slice_base[sl], lane, k);
-        if (st[e] & BCOL_BROKEN) continue;
-        const uint32_t q = h->perm[h->bcol[e]] - a, li = ge - a;
-        if (li < q) {
-          if (dst && r < cap_rows) {
-            dst[2 * r] = li;
-            dst[2 * r + 1] = q;
-          }
-          ++r;
-        }
-      }
-    }
-    if (rows) *rows = r;
-    return PG_OK;
+    return read_conn_from(h, inst, grain, *stp, dst, cap_rows, rows);
   });
 }
 
@@ -1626,6 +1748,8 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
     if (!h->wring) {
       RC(dalloc(h, h->wring, (size_t)h->check_every * wring_stride(h)));
       RC(dalloc(h, h->fring, (size_t)h->check_every * fring_stride(h)));
+      RC(dalloc(h, h->rhits_ring, (size_t)h->check_every * react_chunks(h->CK) * kWflagSpare));
+      RC(dalloc(h, h->rcnt_ring, (size_t)h->check_every * react_chunks(h->CK)));
       HIPCHK(hipMemsetAsync(h->fring, 0, (size_t)h->check_every * fring_stride(h), h->stream));
     }
     HIPCHK(launch_adj_init_break(h->d_bcol, a.break_step, h->E, h->stream));
@@ -1774,6 +1898,111 @@ int pg_kernel_time(pg_handle *h, double *ms, uint64_t *launches) {
   });
 }
 
+// ---------------------------------------------------------- snapshots
+
+int pg_snapshot_take(pg_handle *h, int slot) {
+  return guarded([&]() -> int {
+    if (!h || slot < 0 || slot >= PG_SNAPSHOT_SLOTS) return fail(PG_ERR_INVALID, "bad snapshot slot");
+    HIPCHK(hipSetDevice(h->device));
+    auto &sn = h->snap[slot];
+    if (!sn.alloc) {
+      if (!h->rstream) HIPCHK(hipStreamCreateWithFlags(&h->rstream, hipStreamNonBlocking));
+      RC(dalloc(h, sn.rec, rec_len(h)));
+      RC(dalloc(h, sn.vel, vec_len(h)));
+      RC(dalloc(h, sn.acc, vec_len(h)));
+      RC(dalloc(h, sn.force, vec_len(h)));
+      RC(dalloc(h, sn.wall, (size_t)h->I * 6));
+      RC(dalloc(h, sn.reaction, (size_t)h->I * 18));
+      RC(dalloc(h, sn.bcol, (size_t)h->E));
+      HIPCHK(hipEventCreateWithFlags(&sn.ev, hipEventDisableTiming));
+      sn.alloc = true;
+    }
+    // what pg_read_nodes / pg_read_wall / pg_read_conn read after step t
+    RC(d2d(h, sn.rec, h->rec[h->t & 1], rec_len(h)));
+    RC(d2d(h, sn.vel, h->vel[h->t & 1], vec_len(h)));
+    RC(d2d(h, sn.acc, h->acc, vec_len(h)));
+    RC(d2d(h, sn.force, h->force_out, vec_len(h)));
+    RC(d2d(h, sn.wall, h->wallpos[(h->t + 1) & 1], (size_t)h->I * 6));
+    RC(d2d(h, sn.reaction, h->reaction, (size_t)h->I * 18));
+    RC(d2d(h, sn.bcol, h->d_bcol, (size_t)h->E));
+    HIPCHK(hipEventRecord(sn.ev, h->stream));
+    sn.t = h->t;
+    sn.yshift_grain.resize(h->I);
+    sn.yshift.resize(h->I);
+    for (uint32_t q = 0; q < h->I; ++q) {
+      sn.yshift_grain[q] = h->inst_dev[q].yshift_grain;
+      sn.yshift[q] = h->inst_dev[q].yshift;
+    }
+    sn.bcol_ready = false;
+    sn.taken = true;
+    return PG_OK;
+  });
+}
+
+namespace {
+// The slot's copy, once the handle's stream has made it.
+int snap_ready(pg_handle *h, int slot, pg_handle::Snap *&out) {
+  if (!h || slot < 0 || slot >= PG_SNAPSHOT_SLOTS) return fail(PG_ERR_INVALID, "bad snapshot slot");
+  auto &sn = h->snap[slot];
+  if (!sn.taken) return fail(PG_ERR_STATE, "snapshot slot was never taken");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamWaitEvent(h->rstream, sn.ev, 0));
+  out = &sn;
+  return PG_OK;
+}
+} // namespace
+
+int pg_snapshot_step(pg_handle *h, int slot, uint32_t *t) {
+  return guarded([&]() -> int {
+    if (!h || !t || slot < 0 || slot >= PG_SNAPSHOT_SLOTS || !h->snap[slot].taken)
+      return fail(PG_ERR_INVALID, "bad snapshot slot");
+    *t = h->snap[slot].t;
+    return PG_OK;
+  });
+}
+
+int pg_snapshot_read_nodes(pg_handle *h, int slot, int inst, int field, double *dst) {
+  return guarded([&]() -> int {
+    pg_handle::Snap *sn = nullptr;
+    RC(snap_ready(h, slot, sn));
+    if (!dst || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    return read_nodes_from(h, inst, field, sn->rec, sn->vel, sn->acc, sn->force,
+                           sn->yshift_grain[inst], sn->yshift[inst], h->rstream, dst);
+  });
+}
+
+int pg_snapshot_read_conn(pg_handle *h, int slot, int inst, int grain, uint32_t *dst,
+                          uint64_t cap_rows, uint64_t *rows) {
+  return guarded([&]() -> int {
+    pg_handle::Snap *sn = nullptr;
+    RC(snap_ready(h, slot, sn));
+    if (inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    if (!sn->bcol_ready) {
+      sn->bcol_host.resize(h->E);
+      RC(d2h_on(h->rstream, sn->bcol_host.data(), sn->bcol, (size_t)h->E));
+      sn->bcol_ready = true;
+    }
+    return read_conn_from(h, inst, grain, sn->bcol_host, dst, cap_rows, rows);
+  });
+}
+
+int pg_snapshot_read_wall(pg_handle *h, int slot, int inst, double *wall_info, double *reaction) {
+  return guarded([&]() -> int {
+    pg_handle::Snap *sn = nullptr;
+    RC(snap_ready(h, slot, sn));
+    if (inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const int D = h->dim;
+    if (wall_info) RC(d2h_on(h->rstream, wall_info, sn->wall + (size_t)inst * 6, 6));
+    if (reaction) {
+      double r[18];
+      RC(d2h_on(h->rstream, r, sn->reaction + (size_t)inst * 18, 18));
+      for (int sd = 0; sd < PG_MAX_SIDES; ++sd)
+        for (int k = 0; k < D; ++k) reaction[sd * D + k] = r[sd * 3 + k];
+    }
+    return PG_OK;
+  });
+}
+
 } // extern "C"
 
 extern "C" int pg_device_count(int *n) {
@@ -1785,6 +2014,26 @@ extern "C" int pg_device_count(int *n) {
     if (e == hipErrorNoDevice) return PG_OK;
     if (e != hipSuccess) return fail(PG_ERR_DEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
     *n = c;
+    return PG_OK;
+  });
+}
+
+extern "C" int pg_check_bond(int device, int dim, int check, uint64_t n, const double *xi,
+                             const double *eta, double *s, double *e) {
+  return guarded([&]() -> int {
+    if (!xi || !eta || !s || !e || (dim != 2 && dim != 3)) return fail(PG_ERR_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(device));
+    double *d = nullptr;
+    const size_t nv = n * (size_t)dim, tot = 3 * nv + n;
+    HIPCHK(hipMalloc(&d, (tot ? tot : 1) * sizeof(double)));
+    hipError_t err = hipMemcpy(d, xi, nv * sizeof(double), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(d + nv, eta, nv * sizeof(double), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = launch_bond_geom(dim, check != 0, n, d, d + nv, d + 2 * nv, d + 2 * nv + n, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(s, d + 2 * nv, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(e, d + 2 * nv + n, nv * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (err != hipSuccess) return fail(PG_ERR_DEVICE, std::string("pg_check_bond: ") + hipGetErrorString(err));
     return PG_OK;
   });
 }

@@ -69,6 +69,21 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   g = fma(d, h, g);
   return g;
 }
+// sqrt_nr with the seed taken at max(x, 2^-1000): the same for x >= 2^-1000,
+// and exactly +0 for x = +0 (where rsq(0) = inf would give a NaN).
+__device__ __forceinline__ double sqrt_nr0(double x) {
+  const double r = __builtin_amdgcn_rsq(fmax(x, 0x1p-1000));
+  double g = x * r;
+  double h = r * 0.5;
+  const double e = fma(-h, g, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return g;
+}
 // Ranges where the shortened sequences equal the IEEE ones: denominator in
 // [2^-300, 2^300]; numerator zero (handled by div_r) or in
 // [2^-300, 2^300]; square-root argument in [2^-600, 2^600].
@@ -84,6 +99,24 @@ __device__ __forceinline__ bool sqrt_arg_in_range(double x) {
   const uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(x) >> 32);
   return hi - 0x1A700000u < 0x4B100000u;
 }
+// "Nice" coordinates make the bond loop's per-bond range test unnecessary.
+// A double x is nice if it is zero or 2^-240 <= |x| <= 2^240.  Nice values
+// are integer multiples of 2^-292 (their ulp is at least 2^-240-52), so if
+// every component of X and u is nice:
+//   xi_q = X_jq - X_iq and u_jq - u_iq are zero or multiples of 2^-292 --
+//   the exact difference is such a multiple, and rounding keeps it one
+//   (below 2^-239 it is exact, above, the ulp is at least 2^-291) -- and so
+//   is eta_q = (u_jq - u_iq) + xi_q by the same argument.  Each eta_q is then
+//   zero or at least 2^-292 >= 2^-300 in magnitude; |eta|^2 is zero or at
+//   least 2^-584 >= 2^-600 (and |xi|^2 likewise, nonzero for a bond of
+//   nonzero length); and all of them stay below 2^601 since every
+//   coordinate is at most 4 * 2^240 in magnitude.
+// That is the fast range of bond_geom, except for eta = 0.
+__host__ __device__ __forceinline__ bool nice(double x) {
+  const double m = fabs(x);
+  return m == 0.0 || (m >= 0x1p-240 && m <= 0x1p240);
+}
+
 // a/b through a precomputed reciprocal; a zero numerator returns itself
 // (the IEEE quotient of +-0 by a positive number).
 __device__ __forceinline__ double div_r(double a, double b, double r) {

@@ -157,6 +157,10 @@ struct StepArgs {
   uint32_t *violation;         // first step whose positions broke the skin
   const uint32_t *tbase;       // graph replays: base step added to the launch's step argument
   uint32_t *break_step;        // differentiable runs: step at which each slot's bond broke
+  // [3]: unice[t % 3] == 0 means every displacement component of u^t is
+  // nice() (pg_device.h); step t-1 sets it otherwise, step t-2 clears it
+  uint32_t *unice;
+  uint32_t xi_ok;              // every X component is nice() and no bond has length zero
   // LDS windows (k_step_lds): block b covers blk_slices slices and reads the
   // records of nodes win_lo[b] .. win_lo[b] + win_cnt[b] - 1
   const uint32_t *win_lo;

@@ -29,16 +29,24 @@ hipError_t launch_boxes(int dim, const StepArgs &A, uint32_t t, unsigned long lo
                         hipStream_t s);
 hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gmean,
                              hipStream_t s);
-hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
-                           const double *wterm, const uint8_t *wflag, double *reaction,
-                           hipStream_t s);
-// Reactions of nsteps recorded steps at once: step y reads ring slot
-// (first + y) % slots of the wall terms (wstride apart) and flags (fstride
-// apart) and writes reaction + y * rstride.
-hipError_t launch_reactions(int dim, uint32_t n_insts, const InstDev *insts,
-                            const double *wterm, const uint8_t *wflag, double *reaction,
-                            uint32_t nsteps, uint32_t first, uint32_t slots, size_t wstride,
-                            size_t fstride, size_t rstride, hipStream_t s);
+// Wall reaction of recorded steps (k_wall_compact, then k_reaction).  Step
+// y (0 .. nsteps-1) reads ring slot (first + y) % slots of the wall terms
+// (wstride doubles apart) and flags (fstride bytes apart, a multiple of 16)
+// and writes reaction + y * rstride; hits (hstride = nchunks * 4096 keys per
+// step) and chunk_cnt (cstride per step) are its scratch.
+struct ReactionArgs {
+  const InstDev *insts;
+  uint32_t n_insts;
+  uint32_t nchunks; // ceil(contact keys / 4096)
+  const double *wterm;
+  const uint8_t *wflag;
+  double *reaction;
+  uint32_t *hits;
+  uint32_t *chunk_cnt;
+  uint32_t first, slots;
+  size_t wstride, fstride, rstride, hstride, cstride;
+};
+hipError_t launch_reactions(int dim, const ReactionArgs &R, uint32_t nsteps, hipStream_t s);
 hipError_t launch_fill_box(unsigned long long *p, size_t n, int dim, hipStream_t s);
 hipError_t launch_inst_box(int dim, const StepArgs &A, uint32_t t, unsigned long long *ibox,
                            hipStream_t s);
@@ -61,6 +69,8 @@ hipError_t launch_adj_restore_bcol(uint32_t *bcol, const uint32_t *break_step, u
                                    uint32_t t, hipStream_t s);
 hipError_t launch_adj_init_break(const uint32_t *bcol, uint32_t *break_step, uint64_t n,
                                  hipStream_t s);
+hipError_t launch_bond_geom(int dim, bool check, uint64_t n, const double *xi, const double *eta,
+                            double *s, double *e, hipStream_t st);
 hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
                         hipStream_t s);
 // hipcub exclusive sum; call with tmp=nullptr to size *tmp_bytes.

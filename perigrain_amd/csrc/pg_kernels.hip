@@ -68,63 +68,84 @@ __device__ __forceinline__ void load_nbr(const gbytes *P, const double *Vn, doub
   nb.V = UVOL ? gvol : Vn[j];
 }
 
+// Length |xi|, stretch s = (|eta| - |xi|)/|xi| and direction e = eta/|eta| of
+// one bond (particle2.h:134-146), each a correctly rounded IEEE result.
+// They go through the shortened division and square-root sequences of
+// pg_device.h, which are exact when xin2 = |xi|^2 and n2 = |eta|^2 lie in
+// [2^-600, 2^601) (so n, |xi| lie in [2^-300, 2^301)) and each eta_q is zero
+// or of magnitude >= 2^-300: then every numerator is zero or of magnitude in
+// [2^-352, 2^301] (at most n or max(n, |xi|)) over a denominator in
+// [2^-300, 2^301), where the sequences need no scaling.  The numerator of s,
+// n - |xi|, needs no test: n and |xi| are >= 2^-300, so both are multiples of
+// 2^-352 and their difference is zero or at least 2^-352 in magnitude.  A
+// zero numerator gives a zero quotient whose sign may differ from the IEEE
+// one; a zero of either sign adds nothing to a force and compares equal in
+// the break test, so the results are the same.
+//
+// CHECK: test that range for this bond (bond_fast_range) and use the IEEE
+// operators outside it; only live bonds take the IEEE path (dead slots'
+// values are discarded).  !CHECK: the caller has established the range for
+// every bond of the step (every coordinate of X and u is nice(), no bond has
+// xi = 0) except eta = 0 (coincident nodes), where the IEEE results are
+// n = 0, s = -1 exactly and e = 0/0, a NaN.  The square root of |eta|^2
+// seeds from rsq(max(|eta|^2, 2^-1000)) (sqrt_nr0), which changes nothing in
+// the range and returns n = 0 for |eta|^2 = 0, so s = -|xi|/|xi| = -1
+// exactly and e is a NaN (NaN payloads are not compared: the host's and the
+// device's default NaNs differ anyway).
+__device__ __forceinline__ bool bond_fast_range(double xin2, double n2, const double *eta, int D) {
+  // frexp exponent: zero -> 0, |x| >= 2^-300 -> >= -299
+  int emin = __builtin_amdgcn_frexp_exp(eta[0]);
+  for (int q = 1; q < D; ++q) emin = min(emin, __builtin_amdgcn_frexp_exp(eta[q]));
+  return sqrt_arg_in_range(xin2) && sqrt_arg_in_range(n2) && emin >= -299;
+}
+template <int D, bool CHECK>
+__device__ __forceinline__ void bond_geom(const double *xi, const double *eta, bool live,
+                                          double &s, double *e) {
+  const double xin2 = red<D>(xi, xi);
+  const double n2 = red<D>(eta, eta);
+#if PG_FAST_DIVSQRT
+  double xin = sqrt_nr(xin2), n = CHECK ? sqrt_nr(n2) : sqrt_nr0(n2);
+  const double dn = n - xin;
+  s = div_nr(dn, xin, recip_nr(xin));
+  const double rn = recip_nr(n);
+#pragma unroll
+  for (int q = 0; q < D; ++q) e[q] = div_nr(eta[q], n, rn);
+  if (CHECK) {
+    if (live && !bond_fast_range(xin2, n2, eta, D)) { // outside the fast range: IEEE operators
+      xin = sqrt(xin2);
+      n = sqrt(n2);
+      s = (n - xin) / xin;
+#pragma unroll
+      for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
+    }
+  }
+#else
+  const double xin = sqrt(xin2);
+  const double n = sqrt(n2);
+  s = (n - xin) / xin;
+#pragma unroll
+  for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
+#endif
+}
+
 // One bond of get_peridynamic_force (particle2.h:125-149) and its break test
 // (remove_bonds, particle2.h:207), evaluated for every slot; broken and
 // padding slots (ghost partner, finite values) get a zero weight and add a
 // zero of either sign (the sum starts at +0.0 and can never become -0.0, so
 // adding a zero leaves it bit-identical).  A bond whose stretch exceeds s0
 // gets the broken flag in its column word.
-template <int D>
+template <int D, bool CHECK>
 __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const double *Xo,
                                           const double *ui, double cnot, double snot, bool brk,
                                           uint32_t *slot, double *F, const StepArgs &A,
                                           uint32_t t, bool &broke) {
   const bool live = !(cw & BCOL_BROKEN);
-  double xi[D], eta[D], e[D];
+  double xi[D], eta[D], e[D], s;
 #pragma unroll
   for (int q = 0; q < D; ++q) xi[q] = nb.X[q] - Xo[q];
-  const double xin2 = red<D>(xi, xi);
 #pragma unroll
   for (int q = 0; q < D; ++q) eta[q] = (nb.u[q] - ui[q]) + xi[q];
-  const double n2 = red<D>(eta, eta);
-#if PG_FAST_DIVSQRT
-  // |xi| = sqrt(xin2), |eta| = sqrt(n2), s = (n - |xi|)/|xi|, e = eta/n,
-  // through the shortened sequences (one reciprocal for the three e_q).
-  // Exact when xin2, n2 lie in [2^-600, 2^601) (so n, |xi| lie in
-  // [2^-300, 2^301)) and each eta_q is zero or of magnitude >= 2^-300:
-  // then every numerator is zero or of magnitude in [2^-352, 2^301] (at
-  // most n or max(n, |xi|)) over a denominator in [2^-300, 2^301), where
-  // the sequences need no scaling.  The numerator of s, n - |xi|, needs no
-  // test: n and |xi| are >= 2^-300, so both are multiples of 2^-352 and
-  // their difference is zero or at least 2^-352 in magnitude.  A zero
-  // numerator gives a zero quotient whose sign may differ from the IEEE
-  // one; a zero of either sign adds nothing to F and compares equal in the
-  // break test, so the results are the same.
-  double xin = sqrt_nr(xin2), n = sqrt_nr(n2);
-  const double dn = n - xin;
-  double s = div_nr(dn, xin, recip_nr(xin));
-  const double rn = recip_nr(n);
-#pragma unroll
-  for (int q = 0; q < D; ++q) e[q] = div_nr(eta[q], n, rn);
-  // zero -> 0, |x| >= 2^-300 -> >= -299
-  int emin = __builtin_amdgcn_frexp_exp(eta[0]);
-#pragma unroll
-  for (int q = 1; q < D; ++q) emin = min(emin, __builtin_amdgcn_frexp_exp(eta[q]));
-  const bool ok = sqrt_arg_in_range(xin2) && sqrt_arg_in_range(n2) && emin >= -299;
-  if (live && !ok) { // outside the fast range: IEEE operators
-    xin = sqrt(xin2);
-    n = sqrt(n2);
-    s = (n - xin) / xin;
-#pragma unroll
-    for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
-  }
-#else
-  const double xin = sqrt(xin2);
-  const double n = sqrt(n2);
-  const double s = (n - xin) / xin;
-#pragma unroll
-  for (int q = 0; q < D; ++q) e[q] = eta[q] / n;
-#endif
+  bond_geom<D, CHECK>(xi, eta, live, s, e);
   const double w = live ? (cnot * s) * nb.V : 0.0;
 #pragma unroll
   for (int q = 0; q < D; ++q) F[q] = F[q] + w * e[q];
@@ -322,6 +343,49 @@ __device__ __forceinline__ void wave_box_block(unsigned long long *keys, uint32_
   }
 }
 
+// The PMB bond loop of node i (particle2.h:112-152): its row's slots in
+// order, software-pipelined -- the partner data of the next two bonds and the
+// column words of the two after those are in flight while two bonds are
+// evaluated.
+template <int D, bool CHECK, class RS>
+__device__ __forceinline__ void bond_loop(const StepArgs &A, uint32_t t, uint32_t i, const RS &rs,
+                                          const double *Xo, const double *ui, double cnot,
+                                          double snot, double gvol, bool brk, double *F,
+                                          bool &broke) {
+  const uint32_t sl = i >> 6, lane = i & 63;
+  const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+  // column words of this lane: group q (slots 4q..4q+3) at bc + 256*q
+  uint32_t *bc = A.bcol + base + BGROUP * lane;
+  // the column array has BCOL_TAIL spare slots after the last slice, so
+  // reading two groups past a row's end stays in bounds
+  auto colg = [&](uint32_t q) -> uint4 {
+    const uint4 w = *reinterpret_cast<const uint4 *>(bc + (SLICE * BGROUP) * q);
+    return (BGROUP * q < width) ? w : make_uint4(BCOL_PAD, BCOL_PAD, BCOL_PAD, BCOL_PAD);
+  };
+  Nbr<D> ra0, ra1, rb0, rb1;
+  uint4 cg = colg(0);
+  uint4 cn = colg(1);
+  rs.load(cg.x, i, gvol, ra0);
+  rs.load(cg.y, i, gvol, ra1);
+  // Issue order matters: loads complete in order, so the next column
+  // group (no dependences) goes out before the record loads that need
+  // the group loaded one iteration earlier.
+  for (uint32_t q = 0; BGROUP * q < width; ++q) {
+    uint32_t *sq = bc + (SLICE * BGROUP) * q;
+    rs.load(cg.z, i, gvol, rb0);
+    rs.load(cg.w, i, gvol, rb1);
+    bond_term<D, CHECK>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F, A, t, broke);
+    bond_term<D, CHECK>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F, A, t, broke);
+    const uint4 cnn = colg(q + 2);
+    rs.load(cn.x, i, gvol, ra0);
+    rs.load(cn.y, i, gvol, ra1);
+    bond_term<D, CHECK>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F, A, t, broke);
+    bond_term<D, CHECK>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F, A, t, broke);
+    cg = cn;
+    cn = cnn;
+  }
+}
+
 // ------------------------------------------------------------ step kernel
 
 // One full time step t for every node (timeloop.h:608-878):
@@ -386,6 +450,7 @@ __device__ __forceinline__ void step_housekeeping(const StepArgs &A, uint32_t t,
   unsigned long long *bz = A.bbox[(t + 2) % 3];
   const uint32_t nz = A.n_grains * 2 * D;
   for (uint32_t q = gi; q < nz; q += nthreads) bz[q] = ((q % (2 * D)) < D) ? ~0ull : 0ull;
+  if (gi == 0) A.unice[(t + 2) % 3] = 0; // step t+1 sets it if some u^{t+2} is not nice
   if (gi < A.n_insts * 6) {
     const InstDev &ip = A.insts[gi / 6];
     A.wallpos[(t + 1) & 1][gi] = A.wallpos[t & 1][gi] + ip.speed[gi % 6] * ip.dt;
@@ -431,38 +496,10 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
         const double cnot = gq->cnot, snot = gq->snot;
         const double gvol = UVOL ? gq->vol : 0.0;
         const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
-        const uint32_t sl = i >> 6, lane = i & 63;
-        const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
-        // column words of this lane: group q (slots 4q..4q+3) at bc + 256*q
-        uint32_t *bc = A.bcol + base + BGROUP * lane;
-        // the column array has BCOL_TAIL spare slots after the last slice, so
-        // reading two groups past a row's end stays in bounds
-        auto colg = [&](uint32_t q) -> uint4 {
-          const uint4 w = *reinterpret_cast<const uint4 *>(bc + (SLICE * BGROUP) * q);
-          return (BGROUP * q < width) ? w : make_uint4(BCOL_PAD, BCOL_PAD, BCOL_PAD, BCOL_PAD);
-        };
-        Nbr<D> ra0, ra1, rb0, rb1;
-        uint4 cg = colg(0);
-        uint4 cn = colg(1);
-        rs.load(cg.x, i, gvol, ra0);
-        rs.load(cg.y, i, gvol, ra1);
-        // Issue order matters: loads complete in order, so the next column
-        // group (no dependences) goes out before the record loads that need
-        // the group loaded one iteration earlier.
-        for (uint32_t q = 0; BGROUP * q < width; ++q) {
-          uint32_t *sq = bc + (SLICE * BGROUP) * q;
-          rs.load(cg.z, i, gvol, rb0);
-          rs.load(cg.w, i, gvol, rb1);
-          bond_term<D>(cg.x, ra0, Xo, ui, cnot, snot, brk, sq + 0, F, A, t, broke);
-          bond_term<D>(cg.y, ra1, Xo, ui, cnot, snot, brk, sq + 1, F, A, t, broke);
-          const uint4 cnn = colg(q + 2);
-          rs.load(cn.x, i, gvol, ra0);
-          rs.load(cn.y, i, gvol, ra1);
-          bond_term<D>(cg.z, rb0, Xo, ui, cnot, snot, brk, sq + 2, F, A, t, broke);
-          bond_term<D>(cg.w, rb1, Xo, ui, cnot, snot, brk, sq + 3, F, A, t, broke);
-          cg = cn;
-          cn = cnn;
-        }
+        if (A.unice[t % 3] == 0 && A.xi_ok) // the same for every lane of the launch
+          bond_loop<D, false>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
+        else
+          bond_loop<D, true>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
         if (broke) A.brk_new[i] = 1;
       }
     }
@@ -483,7 +520,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     load_vec<D>(vprev, ns, i, vi);
     load_vec<D>(A.acc, ns, i, ai);
     const uint32_t key = A.ckey[i];
-    bool wall_hit = false;
+    uint32_t wmask = 0; // bit s: the node touched wall side s
     double wt[6][4];
 #pragma unroll
     for (int s = 0; s < 6; ++s)
@@ -543,17 +580,17 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
         if (!within_interior<D>(boxg, wp, ip.Rc)) {
           const double vol = A.V[i];
           if (D == 2) {
-            wall_hit |= wall_side<D, double>(xc[1] - wp[3], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[3], vi, F, wt[3]);
-            wall_hit |= wall_side<D, double>(xc[0] - wp[0], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[0], vi, F, wt[0]);
-            wall_hit |= wall_side<D, double>(xc[0] - wp[1], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[1], vi, F, wt[1]);
-            wall_hit |= wall_side<D, double>(xc[1] - wp[2], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[2], vi, F, wt[2]);
+            wmask |= wall_side<D, double>(xc[1] - wp[3], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[3], vi, F, wt[3]) ? (1u << 3) : 0u;
+            wmask |= wall_side<D, double>(xc[0] - wp[0], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[0], vi, F, wt[0]) ? (1u << 0) : 0u;
+            wmask |= wall_side<D, double>(xc[0] - wp[1], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[1], vi, F, wt[1]) ? (1u << 1) : 0u;
+            wmask |= wall_side<D, double>(xc[1] - wp[2], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[2], vi, F, wt[2]) ? (1u << 2) : 0u;
           } else {
-            wall_hit |= wall_side<D, double>(xc[0] - wp[0], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[0], vi, F, wt[0]);
-            wall_hit |= wall_side<D, double>(xc[0] - wp[3], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[3], vi, F, wt[3]);
-            wall_hit |= wall_side<D, double>(xc[1] - wp[1], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[1], vi, F, wt[1]);
-            wall_hit |= wall_side<D, double>(xc[1] - wp[4], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[4], vi, F, wt[4]);
-            wall_hit |= wall_side<D, double>(xc[2] - wp[2], 2, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[2], vi, F, wt[2]);
-            wall_hit |= wall_side<D, double>(xc[2] - wp[5], 2, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[5], vi, F, wt[5]);
+            wmask |= wall_side<D, double>(xc[0] - wp[0], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[0], vi, F, wt[0]) ? (1u << 0) : 0u;
+            wmask |= wall_side<D, double>(xc[0] - wp[3], 0, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[3], vi, F, wt[3]) ? (1u << 3) : 0u;
+            wmask |= wall_side<D, double>(xc[1] - wp[1], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[1], vi, F, wt[1]) ? (1u << 1) : 0u;
+            wmask |= wall_side<D, double>(xc[1] - wp[4], 1, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[4], vi, F, wt[4]) ? (1u << 4) : 0u;
+            wmask |= wall_side<D, double>(xc[2] - wp[2], 2, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[2], vi, F, wt[2]) ? (1u << 2) : 0u;
+            wmask |= wall_side<D, double>(xc[2] - wp[5], 2, ip, ip.Kn, ip.mu, gp.rho, vol, ip.speed[5], vi, F, wt[5]) ? (1u << 5) : 0u;
           }
         }
       }
@@ -647,6 +684,10 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
       for (int k = 0; k < D; ++k) un[k] = ui[k];
     }
     rec_set_u<D>(A.rec[(t + 1) & 1], A.ns, i, un);
+    bool unice = true;
+#pragma unroll
+    for (int k = 0; k < D; ++k) unice = unice && nice(un[k]);
+    if (!unice) atomicOr(&A.unice[(t + 1) % 3], 1u); // rare: the next step checks each bond
 #pragma unroll
     for (int k = 0; k < D; ++k) xn[k] = Xc[k] + un[k];
     if (info & NI_CONTACT) {
@@ -658,12 +699,11 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     if (RECORD) {
       store_vec<D>(A.force_out, ns, i, F);
       if (key != 0xFFFFFFFFu) {
-        A.wflag[key] = wall_hit ? 1 : 0;
-        if (wall_hit) {
-          const int nsd = (D == 2) ? 4 : 6;
-          for (int s = 0; s < nsd; ++s)
+        A.wflag[key] = (uint8_t)wmask;
+        const int nsd = (D == 2) ? 4 : 6;
+        for (int s = 0; s < nsd; ++s)
+          if ((wmask >> s) & 1u)
             for (int q = 0; q < 4; ++q) A.wterm[((size_t)key * 6 + s) * 4 + q] = wt[s][q];
-        }
       }
     }
   }
@@ -869,93 +909,198 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
 }
 
 // Wall reaction of recorded steps, summed in the reference's sequential
-// order (grain, node in contact order, then side): one block per instance
-// (x) and step (y).  Step y's wall terms and flags are in ring slot
-// (first + y) % slots, wstride/fstride apart; its reaction goes to
-// reaction + y * rstride.  contact.h:339,374,411-412,428 via
-// timeloop.h:753-769.
-template <int D>
-__global__ __launch_bounds__(256) void k_reaction(const InstDev *insts, const double *wterm,
-                                                  const uint8_t *wflag, double *reaction,
-                                                  uint32_t first, uint32_t slots, size_t wstride,
-                                                  size_t fstride, size_t rstride) {
-  const uint32_t slot = (first + blockIdx.y) % slots;
-  wterm += slot * wstride;
-  wflag += slot * fstride;
-  reaction += blockIdx.y * rstride;
-  constexpr int CH = (int)kWflagSpare, PER = CH / 256;
-  __shared__ uint32_t list[CH];
+// order (grain, node in contact order, then side; contact.h:339,374,
+// 411-412,428 via timeloop.h:753-769), in two launches.
+//
+// k_wall_compact: the keys whose node touched a wall, compacted in key
+// order, in parallel over chunks of REACT_CHUNK keys: chunk b of recorded
+// step y writes its hit keys, ascending, to hits[y][b*REACT_CHUNK ...] and
+// their number to chunk_cnt[y][b].  One block per (chunk, step).
+constexpr uint32_t REACT_CHUNK = 4096;           // keys per chunk: 16 per thread
+constexpr uint32_t REACT_PIECE = 256;            // hits staged in LDS at a time
+static_assert(REACT_CHUNK == (uint32_t)kWflagSpare, "flag loads run one chunk past the keys");
+
+__global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
+  const uint32_t slot = (R.first + blockIdx.y) % R.slots;
+  const uint8_t *wflag = R.wflag + slot * R.fstride;
+  uint32_t *hits = R.hits + blockIdx.y * R.hstride + (size_t)blockIdx.x * REACT_CHUNK;
   __shared__ uint32_t wtot[4];
-  const InstDev ip = insts[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ns = ip.nsides;
-  double acc = 0.0;
-  const int s = tid / D, c = tid % D;
-  const int dir = (D == 2) ? ((s < 2) ? 0 : 1) : (s % 3);
-  for (uint32_t base = ip.ckey_begin; base < ip.ckey_end; base += CH) {
-    // the keys of this chunk with a wall term, compacted in key order: each
-    // thread takes PER consecutive keys; a block-wide exclusive scan of the
-    // per-thread counts (wavefront shuffles, then the four wavefront totals)
-    // gives each thread its first output position
-    uint32_t loc[PER];
-    uint32_t cnt = 0;
-    // (wflag has a chunk of zero bytes past its end, so every load is in
-    // bounds and none waits for a branch)
-    uint8_t f[PER];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t k0 = blockIdx.x * REACT_CHUNK + tid * 16;
+  // 16 flags in one load (the flag array has a chunk of zero bytes past the
+  // last key, and ring slots start on 16-byte boundaries)
+  const uint4 f4 = *reinterpret_cast<const uint4 *>(wflag + k0);
+  const uint32_t w[4] = {f4.x, f4.y, f4.z, f4.w};
+  uint32_t cnt = 0;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) f[q] = wflag[base + tid * PER + q];
+  for (int q = 0; q < 16; ++q) cnt += ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) ? 1u : 0u;
+  uint32_t v = cnt;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const uint32_t key = base + tid * PER + q;
-      const bool on = key < ip.ckey_end && f[q];
-      loc[q] = on ? key : 0xFFFFFFFFu;
-      cnt += on ? 1 : 0;
-    }
-    uint32_t v = cnt;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, 64);
+    if (lane >= (uint32_t)off) v += u;
+  }
+  if (lane == 63) wtot[wave] = v;
+  __syncthreads();
+  uint32_t pos = v - cnt, total = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q) {
+    if (q < wave) pos += wtot[q];
+    total += wtot[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) hits[pos++] = k0 + q;
+  if (tid == 0) R.chunk_cnt[blockIdx.y * R.cstride + blockIdx.x] = total;
+}
+
+// k_reaction: one block per recorded step.  The compacted hits are taken in
+// pieces of REACT_PIECE keys.  The whole block stages a piece: each hit's
+// side mask (the flag byte: bit s for a touched side s) and the four terms
+// of every side it touched -- normal, damping, friction along the side's two
+// tangents -- go into one list per side, in key order (a packed block scan
+// of the per-side counts gives the positions).  Then one thread per
+// (instance, side, component) subtracts its side's terms in key order:
+// terms 0 and 1 along the side's direction, term 2 or 3 across it.  A node
+// that did not touch a side adds nothing to it (in the reference that side
+// is not updated; here it is not visited).  Running sums live in the output
+// between pieces.
+template <int D>
+__global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
+  const uint32_t slot = (R.first + blockIdx.y) % R.slots;
+  const double *wterm = R.wterm + slot * R.wstride;
+  const uint8_t *wflag = R.wflag + slot * R.fstride;
+  const uint32_t *hits = R.hits + blockIdx.y * R.hstride;
+  const uint32_t *ccnt = R.chunk_cnt + blockIdx.y * R.cstride;
+  double *reaction = R.reaction + blockIdx.y * R.rstride;
+  constexpr int NS = (D == 2) ? 4 : 6;
+  constexpr uint32_t PER_INST = NS * D;
+  __shared__ uint32_t s_off[257];
+  __shared__ uint32_t wtot[4];
+  __shared__ unsigned long long wtot64[4];
+  __shared__ uint32_t s_key[NS][REACT_PIECE];
+  __shared__ double2 s_side[NS][REACT_PIECE][2]; // terms (0, 1), (2, 3) of one side
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t n_assign = R.n_insts * PER_INST;
+  for (uint32_t a = tid; a < n_assign; a += blockDim.x)
+    reaction[(size_t)(a / PER_INST) * 18 + ((a / D) % NS) * 3 + a % D] = 0.0;
+  for (uint32_t g0 = 0; g0 < R.nchunks; g0 += 256) {
+    // offsets of this group's chunks in the key-ordered list of hits
+    const uint32_t c = (g0 + tid < R.nchunks) ? ccnt[g0 + tid] : 0u;
+    uint32_t v = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t u = __shfl_up(v, off, 64);
-      if (lane >= off) v += u;
+      if (lane >= (uint32_t)off) v += u;
     }
     if (lane == 63) wtot[wave] = v;
     __syncthreads();
-    uint32_t w = v - cnt, total = 0;
+    uint32_t excl = v - c, total = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (q < wave) w += wtot[q];
+    for (uint32_t q = 0; q < 4; ++q) {
+      if (q < wave) excl += wtot[q];
       total += wtot[q];
     }
-#pragma unroll
-    for (int q = 0; q < PER; ++q)
-      if (loc[q] != 0xFFFFFFFFu) list[w++] = loc[q];
+    s_off[tid] = excl;
+    if (tid == 0) s_off[256] = total;
     __syncthreads();
-    // one thread per (side, component) sums its terms in key order: terms
-    // 0 and 1 along the side's direction, term 2 or 3 across it.  Eight
-    // keys' terms are loaded together, then subtracted in order (a single
-    // term is followed by subtracting +0.0, which leaves every value as it is).
-    if (tid < ns * D) {
-      const int o1 = (c == dir) ? 0 : ((c == (dir + 1) % D) ? 2 : 3);
-      const bool two = (c == dir);
-      for (uint32_t q = 0; q < total; q += 8) {
-        double a1[8], a2[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const uint32_t key = list[(q + k < total) ? q + k : q];
-          const double *p = wterm + ((size_t)key * 6 + s) * 4;
-          a1[k] = p[o1];
-          a2[k] = p[1];
+    for (uint32_t p0 = 0; p0 < total; p0 += REACT_PIECE) {
+      const uint32_t np = min(REACT_PIECE, total - p0);
+      uint32_t key = 0, mask = 0;
+      double2 t[NS][2];
+      if (tid < np) {
+        // chunk of hit e: the last chunk whose offset is <= e
+        const uint32_t e = p0 + tid;
+        uint32_t lo = 0, hi = 256;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_off[mid] <= e) lo = mid;
+          else hi = mid;
         }
+        key = hits[(size_t)(g0 + lo) * REACT_CHUNK + (e - s_off[lo])];
+        mask = wflag[key];
+        const double2 *src = reinterpret_cast<const double2 *>(wterm + (size_t)key * 24);
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (q + k < total) {
-            acc = acc - a1[k];
-            acc = acc - (two ? a2[k] : 0.0);
+        for (int sd = 0; sd < NS; ++sd)
+          if ((mask >> sd) & 1u) {
+            t[sd][0] = src[2 * sd];
+            t[sd][1] = src[2 * sd + 1];
           }
       }
+      // positions in the per-side lists: a block scan of the per-side
+      // counts packed 9 bits per side (a piece has at most 256 hits)
+      unsigned long long pk = 0;
+#pragma unroll
+      for (int sd = 0; sd < NS; ++sd) pk |= (unsigned long long)((mask >> sd) & 1u) << (9 * sd);
+      unsigned long long w = pk;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long u = __shfl_up(w, off, 64);
+        if (lane >= (uint32_t)off) w += u;
+      }
+      if (lane == 63) wtot64[wave] = w;
+      __syncthreads();
+      unsigned long long pos = w - pk, tot = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) {
+        if (q < wave) pos += wtot64[q];
+        tot += wtot64[q];
+      }
+#pragma unroll
+      for (int sd = 0; sd < NS; ++sd)
+        if ((mask >> sd) & 1u) {
+          const uint32_t ps = (uint32_t)(pos >> (9 * sd)) & 511u;
+          s_key[sd][ps] = key;
+          s_side[sd][ps][0] = t[sd][0];
+          s_side[sd][ps][1] = t[sd][1];
+        }
+      __syncthreads();
+      for (uint32_t a = tid; a < n_assign; a += blockDim.x) {
+        const uint32_t inst = a / PER_INST, sd = (a / D) % NS, cp = a % D;
+        const uint32_t nside = (uint32_t)(tot >> (9 * sd)) & 511u;
+        const int dir = (D == 2) ? ((sd < 2) ? 0 : 1) : (int)(sd % 3);
+        const int o1 = ((int)cp == dir) ? 0 : (((int)cp == (dir + 1) % D) ? 2 : 3);
+        const bool two = ((int)cp == dir);
+        // this instance's entries of the side's list (keys ascend, instances
+        // own consecutive key ranges)
+        const uint32_t kb = R.insts[inst].ckey_begin, ke = R.insts[inst].ckey_end;
+        uint32_t lo = 0, hi = nside;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_key[sd][mid] < kb) lo = mid + 1;
+          else hi = mid;
+        }
+        uint32_t e1 = lo, e2 = nside;
+        while (e1 < e2) {
+          const uint32_t mid = (e1 + e2) >> 1;
+          if (s_key[sd][mid] < ke) e1 = mid + 1;
+          else e2 = mid;
+        }
+        if (lo == e1) continue;
+        const double *base = reinterpret_cast<const double *>(&s_side[sd][0][0]);
+        double *out = reaction + (size_t)inst * 18 + sd * 3 + cp;
+        double acc = *out;
+        // eight entries' terms are read before they are subtracted in order
+        for (uint32_t e = lo; e < e1; e += 8) {
+          double a1[8], a2[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const uint32_t q = (e + k < e1) ? e + k : e;
+            a1[k] = base[q * 4 + o1];
+            a2[k] = base[q * 4 + 1];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (e + k < e1) {
+              acc = acc - a1[k];
+              if (two) acc = acc - a2[k];
+            }
+        }
+        *out = acc;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  if (tid < ns * D) reaction[(size_t)blockIdx.x * 18 + s * 3 + c] = acc;
 }
 
 // ------------------------------------------------------- neighbour list
@@ -1217,6 +1362,27 @@ __global__ void k_arith(uint64_t n, const double *a, const double *b, double *q,
 #endif
 }
 
+// Diagnostics: bond_geom itself -- the bond loop's length, stretch and
+// direction arithmetic, its range test (CHECK) or the nice-input shortcut
+// (!CHECK) -- on given xi and eta vectors ([n][D]), so tests can compare its
+// rounding with the host's IEEE operators at the edges of its ranges.
+template <int D, bool CHECK>
+__global__ void k_bond_geom(uint64_t n, const double *xi, const double *eta, double *s,
+                            double *e) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[D], y[D], ee[D], ss;
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    x[q] = xi[i * D + q];
+    y[q] = eta[i * D + q];
+  }
+  bond_geom<D, CHECK>(x, y, true, ss, ee);
+  s[i] = ss;
+#pragma unroll
+  for (int q = 0; q < D; ++q) e[i * D + q] = ee[q];
+}
+
 // ------------------------------------------------------------- launchers
 
 static inline uint32_t nblk(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
@@ -1294,25 +1460,12 @@ hipError_t launch_grain_mean(int dim, const StepArgs &A, uint32_t t, double *gme
   return hipGetLastError();
 }
 
-hipError_t launch_reaction(int dim, uint32_t n_insts, const InstDev *insts,
-                           const double *wterm, const uint8_t *wflag, double *reaction,
-                           hipStream_t s) {
-  return launch_reactions(dim, n_insts, insts, wterm, wflag, reaction, 1, 0, 1, 0, 0,
-                          (size_t)n_insts * 18, s);
-}
-
-hipError_t launch_reactions(int dim, uint32_t n_insts, const InstDev *insts,
-                            const double *wterm, const uint8_t *wflag, double *reaction,
-                            uint32_t nsteps, uint32_t first, uint32_t slots, size_t wstride,
-                            size_t fstride, size_t rstride, hipStream_t s) {
+hipError_t launch_reactions(int dim, const ReactionArgs &R, uint32_t nsteps, hipStream_t s) {
   if (nsteps == 0) return hipSuccess;
-  const dim3 g(n_insts, nsteps);
-  if (dim == 2)
-    hipLaunchKernelGGL(k_reaction<2>, g, dim3(256), 0, s, insts, wterm, wflag, reaction, first,
-                       slots, wstride, fstride, rstride);
-  else
-    hipLaunchKernelGGL(k_reaction<3>, g, dim3(256), 0, s, insts, wterm, wflag, reaction, first,
-                       slots, wstride, fstride, rstride);
+  if (R.nchunks > 0)
+    hipLaunchKernelGGL(k_wall_compact, dim3(R.nchunks, nsteps), dim3(256), 0, s, R);
+  if (dim == 2) hipLaunchKernelGGL(k_reaction<2>, dim3(1, nsteps), dim3(256), 0, s, R);
+  else hipLaunchKernelGGL(k_reaction<3>, dim3(1, nsteps), dim3(256), 0, s, R);
   return hipGetLastError();
 }
 
@@ -1394,6 +1547,20 @@ hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q,
                         hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_arith, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, a, b, q, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_bond_geom(int dim, bool check, uint64_t n, const double *xi, const double *eta,
+                            double *s, double *e, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const dim3 g((uint32_t)((n + 255) / 256)), b(256);
+  if (dim == 2) {
+    if (check) hipLaunchKernelGGL((k_bond_geom<2, true>), g, b, 0, st, n, xi, eta, s, e);
+    else hipLaunchKernelGGL((k_bond_geom<2, false>), g, b, 0, st, n, xi, eta, s, e);
+  } else {
+    if (check) hipLaunchKernelGGL((k_bond_geom<3, true>), g, b, 0, st, n, xi, eta, s, e);
+    else hipLaunchKernelGGL((k_bond_geom<3, false>), g, b, 0, st, n, xi, eta, s, e);
+  }
   return hipGetLastError();
 }
 

@@ -194,6 +194,45 @@ class Stepper:
                                         g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return g
 
+    # ------------------------------------------------------------ snapshots
+    def snapshot_take(self, slot=0):
+        """Copy the last completed step's snapshot state into slot `slot` on
+        the device, in stream order (pg_snapshot_take); returns at once."""
+        check(lib().pg_snapshot_take(self._h, int(slot)))
+
+    def snapshot_step(self, slot=0):
+        v = ctypes.c_uint32()
+        check(lib().pg_snapshot_step(self._h, int(slot), ctypes.byref(v)))
+        return int(v.value)
+
+    def snapshot_read(self, field, slot=0, inst=0):
+        """A node field of a taken snapshot slot; may be called from another
+        thread while this handle steps."""
+        n, _, _ = self.sizes(inst)
+        out = np.empty((n, self.dim), dtype=np.float64)
+        check(lib().pg_snapshot_read_nodes(self._h, int(slot), inst, _ffi.FIELDS[field],
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def snapshot_conn(self, grain, slot=0, inst=0):
+        rows = ctypes.c_uint64()
+        check(lib().pg_snapshot_read_conn(self._h, int(slot), inst, grain, None, 0,
+                                          ctypes.byref(rows)))
+        out = np.empty((rows.value, 2), dtype=np.uint32)
+        check(lib().pg_snapshot_read_conn(self._h, int(slot), inst, grain,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                          rows.value, ctypes.byref(rows)))
+        return out
+
+    def snapshot_wall(self, slot=0, inst=0):
+        info = np.zeros(6)
+        react = np.zeros(6 * self.dim)
+        check(lib().pg_snapshot_read_wall(self._h, int(slot), inst,
+                                          info.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          react.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        ns = 4 if self.dim == 2 else 6
+        return info[:ns].copy(), react.reshape(6, self.dim)[:ns].copy()
+
     def kernel_time(self):
         """(milliseconds, launches) of step-kernel runs so far (pg_kernel_time)."""
         ms = ctypes.c_double()

@@ -79,3 +79,46 @@ def test_kernel_timer_counts_step_launches():
         g.step(130)
         ms, n = g.kernel_time()
         assert 0 < n <= 130 and ms > 0
+
+
+def test_snapshot_slots_read_while_stepping():
+    # a snapshot slot holds exactly what the plain readers gave right after
+    # its step, read back on another thread while the handle steps on
+    import threading
+    s, p = setups.two_disk_collision(seed=4, gap=1e-5, speed=50.0)
+    nodes = s.grain_nodes(0)
+    s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))  # bonds break
+    fields = ("CurrPos", "vel", "acc", "force", "disp")
+    with Stepper(s, p, device=0) as g, Stepper(s, p, device=0) as ref:
+        want = {}
+        for slot, t in ((0, 40), (1, 90)):
+            g.step(t - g.t)
+            ref.step(t - ref.t)
+            g.snapshot_take(slot)
+            want[slot] = ({f: ref.read(f) for f in fields},
+                          [ref.conn(k) for k in range(s.n_grains)], ref.wall())
+        got = {}
+
+        def reader():
+            for slot in (0, 1):
+                got[slot] = ({f: g.snapshot_read(f, slot) for f in fields},
+                             [g.snapshot_conn(k, slot) for k in range(s.n_grains)],
+                             g.snapshot_wall(slot), g.snapshot_step(slot))
+
+        th = threading.Thread(target=reader)
+        th.start()
+        g.step(200)  # the live state moves on meanwhile
+        th.join()
+        assert g.t == 290
+        for slot, t in ((0, 40), (1, 90)):
+            wf, wc, (ww, wr) = want[slot]
+            gf, gc, (gw, gr), gt = got[slot]
+            assert gt == t
+            for f in fields:
+                assert np.array_equal(gf[f], wf[f]), (slot, f)
+            for k in range(s.n_grains):
+                assert np.array_equal(gc[k], wc[k]), (slot, k)
+            assert np.array_equal(gw, ww) and np.array_equal(gr, wr)
+        assert len(want[1][1][0]) < len(want[0][1][0]), "bonds should break between the slots"
+        with pytest.raises(_ffi.PgError):
+            g.snapshot_read("vel", 2)

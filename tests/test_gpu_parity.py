@@ -75,6 +75,87 @@ def test_division_and_sqrt_round_like_host():
     assert np.array_equal(r, np.sqrt(a))
 
 
+def _red(a, b):
+    if a.shape[1] == 2:
+        return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]
+    return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+
+def _bond_host(xi, eta):
+    """The reference's bond arithmetic (particle2.h:134-146) with the host's
+    IEEE operators: s = (|eta| - |xi|)/|xi|, e = eta/|eta|."""
+    with np.errstate(all="ignore"):
+        xin = np.sqrt(_red(xi, xi))
+        n = np.sqrt(_red(eta, eta))
+        return (n - xin) / xin, eta / n[:, None]
+
+
+def _bond_device(dim, check, xi, eta):
+    n = len(xi)
+    s = np.empty(n)
+    e = np.empty((n, dim))
+    dp = lambda x: x.ctypes.data_as(_ffi.ctypes.POINTER(_ffi.ctypes.c_double))
+    xi, eta = np.ascontiguousarray(xi), np.ascontiguousarray(eta)
+    _ffi.check(_ffi.lib().pg_check_bond(0, dim, check, n, dp(xi), dp(eta), dp(s), dp(e)))
+    return s, e
+
+
+def _same(a, b):
+    # equal values (a zero quotient's sign may differ, which no force or
+    # break test can see) or NaN on both sides
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bond_arithmetic_at_its_range_edges(dim):
+    """The bond loop's own arithmetic (bond_geom: shortened division and
+    square-root sequences behind its range test, IEEE operators outside)
+    against the host's IEEE results, bit for bit, on inputs chosen at the
+    edges of the range test: |xi|^2 and |eta|^2 around 2^-600 and 2^601,
+    components of eta around 2^-300, subnormals, zeros, and wide random
+    exponents."""
+    rng = np.random.default_rng(17 + dim)
+    n = 1 << 18
+
+    def comp(lo, hi, size):
+        m = rng.uniform(1.0, 2.0, size) * rng.choice([-1.0, 1.0], size)
+        return np.ldexp(m, rng.integers(lo, hi + 1, size))
+
+    sets = []
+    for lo, hi in ((-303, -297), (297, 303), (-560, 560), (-1074, -1000), (-320, -280),
+                   (280, 320)):
+        sets.append((comp(lo, hi, (n, dim)), comp(lo, hi, (n, dim))))
+    # big and tiny components mixed within one vector, and exact zeros
+    a = comp(-305, 305, (n, dim))
+    b = comp(-305, 305, (n, dim))
+    a[rng.uniform(size=a.shape) < 0.2] = 0.0
+    b[rng.uniform(size=b.shape) < 0.2] = 0.0
+    sets.append((a, b))
+    xi = np.concatenate([x for x, _ in sets])
+    eta = np.concatenate([y for _, y in sets])
+    s_d, e_d = _bond_device(dim, 1, xi, eta)
+    s_h, e_h = _bond_host(xi, eta)
+    assert np.all(_same(s_d, s_h)), f"stretch differs at {np.argwhere(~_same(s_d, s_h))[:3].ravel()}"
+    assert np.all(_same(e_d, e_h)), "direction differs"
+    # the shortcut taken when every coordinate is nice (zero or of magnitude
+    # in [2^-240, 2^240]): xi and eta as the kernel forms them from nice X, u
+    X = comp(-240, 239, (2, n, dim))
+    U = comp(-240, 239, (2, n, dim))
+    X[rng.uniform(size=X.shape) < 0.1] = 0.0
+    U[rng.uniform(size=U.shape) < 0.3] = 0.0
+    for k in range(6):  # the extremes
+        X[k % 2, k::97, k % dim] = np.ldexp(1.0, 239 - 478 * (k % 2)) * (1 + 0.1 * k)
+    xi2 = X[1] - X[0]
+    eta2 = (U[1] - U[0]) + xi2
+    keep = np.any(xi2 != 0, axis=1)
+    xi2, eta2 = xi2[keep], eta2[keep]
+    eta2[::1009] = 0.0  # coincident nodes: s = -1, e NaN
+    s_d, e_d = _bond_device(dim, 0, xi2, eta2)
+    s_h, e_h = _bond_host(xi2, eta2)
+    assert np.all(s_d[::1009] == -1.0) and np.all(np.isnan(e_d[::1009]))
+    assert np.all(_same(s_d, s_h)) and np.all(_same(e_d, e_h))
+
+
 def test_two_disk_collision_bit_exact(oracle_threads):
     s, p = setups.two_disk_collision(seed=3, gap=5e-5)
     g, o = _both(s, p, oracle_threads)
