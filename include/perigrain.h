@@ -233,6 +233,21 @@ int pg_read_contacts(pg_handle *h, int inst, uint32_t *dst, uint64_t cap_pairs,
 #define PG_GRAIN_STATS(dim) (5 * (dim) + 1)
 int pg_grain_stats(pg_handle *h, int inst, double *out);
 
+/* pg_grain_stats without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory at the time of the
+ * call (tight serving loops that sample the per-grain series every few
+ * steps, harnesses that forbid hipMalloc).  Same kernel and launch, same
+ * output, bit for bit; the per-grain results are staged in a static device
+ * array of PG_NOALLOC_STATS_DOUBLES doubles (2 MiB) that the library holds on
+ * each device it runs on until the process ends, instead of in a buffer it
+ * allocates.  Instances with more than PG_NOALLOC_STATS_DOUBLES /
+ * PG_GRAIN_STATS(dim) grains are rejected with PG_ERR_INVALID.  Every call
+ * on a device shares that device's array: calls that overlap on one device
+ * (from several host threads) race and must be serialised by the caller.
+ * Each call writes every value it returns, so nothing is left from an
+ * earlier call. */
+#define PG_NOALLOC_STATS_DOUBLES (1 << 18)
+int pg_grain_stats_noalloc(pg_handle *h, int inst, double *out);
 
 /* Stream (hipStream_t) the handle enqueues its kernels on. */
 int pg_stream(pg_handle *h, void **stream);

@@ -187,6 +187,7 @@ def lib():
         "pg_read_wall": [H, ctypes.c_int, _dp, _dp],
         "pg_read_bond_intact": [H, ctypes.c_int, _u8p],
         "pg_grain_stats": [H, ctypes.c_int, _dp],
+        "pg_grain_stats_noalloc": [H, ctypes.c_int, _dp],
         "pg_read_conn": [H, ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p],
         "pg_read_contacts": [H, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p],
         "pg_stream": [H, ctypes.POINTER(ctypes.c_void_p)],

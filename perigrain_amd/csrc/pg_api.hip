@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -78,6 +79,19 @@ struct InstHost {
   bool reset_done = false;
 };
 
+} // namespace
+
+// Staging array of pg_grain_stats_noalloc: the per-grain results go here
+// instead of into a buffer allocated at call time.  Each device holds its own
+// copy; host code never passes the array's name to a kernel or a copy (on the
+// host it is only a stand-in), but the device address that
+// hipGetSymbolAddress returns on that device, kept in g_scratch_dev.
+static __device__ __align__(256) double scratch[PG_NOALLOC_STATS_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
 } // namespace
 
 struct pg_handle {
@@ -1579,6 +1593,42 @@ int pg_grain_stats(pg_handle *h, int inst, double *out) {
     }
     HIPCHK(launch_grain_stats(D, h->A, h->t, ih.grain_begin, ng, h->d_stats, h->stream));
     RC(d2h(h, out, h->d_stats, nv));
+    return PG_OK;
+  });
+}
+
+int pg_grain_stats_noalloc(pg_handle *h, int inst, double *out) {
+  return guarded([&]() -> int {
+    if (!h || !out || inst < 0 || inst >= (int)h->I) return fail(PG_ERR_INVALID, "bad argument");
+    const int D = h->dim;
+    const InstHost &ih = h->inst[inst];
+    const uint32_t ng = ih.grain_end - ih.grain_begin;
+    const size_t nv = (size_t)PG_GRAIN_STATS(D) * ng;
+    if (nv > (size_t)PG_NOALLOC_STATS_DOUBLES)
+      return fail(PG_ERR_INVALID, "pg_grain_stats_noalloc: the instance has " + std::to_string(ng) +
+                                      " grains; its static staging array holds " +
+                                      std::to_string(PG_NOALLOC_STATS_DOUBLES / PG_GRAIN_STATS(D)) +
+                                      " (pg_grain_stats takes any number)");
+    if (h->device < 0 || h->device >= kMaxDevices)
+      return fail(PG_ERR_INVALID, "pg_grain_stats_noalloc: device index beyond " +
+                                      std::to_string(kMaxDevices - 1));
+    HIPCHK(hipSetDevice(h->device));
+    double *stage = nullptr;
+    {
+      // once per device, with that device current: the device address of
+      // this device's copy of the array
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      if (!g_scratch_dev[h->device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PG_ERR_DEVICE, "hipGetSymbolAddress returned no address");
+        g_scratch_dev[h->device] = static_cast<double *>(p);
+      }
+      stage = g_scratch_dev[h->device];
+    }
+    // every value of the output range is written by the kernel: nothing to clear
+    HIPCHK(launch_grain_stats(D, h->A, h->t, ih.grain_begin, ng, stage, h->stream));
+    RC(d2h(h, out, stage, nv));
     return PG_OK;
   });
 }
