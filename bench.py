@@ -39,6 +39,7 @@ each advanced the same number of steps in turn).  Both are timed on the host
 clock over whole pg_step calls.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -56,7 +57,12 @@ def algorithmic_bytes(n_nodes, n_slots, dim):
     return n_nodes * (8 * (8 * dim + 1) + 16) + 5 * n_slots
 
 
-def cpu_baseline(setup, params, threads, timed_steps=2):
+def cpu_baseline(setup, params, threads, timed_steps=3, one_thread_grains=8):
+    """The oracle (the reference algorithm restated in C++, OpenMP over
+    grains) on the host: `timed_steps` steps of the whole workload after one
+    warm-up step with `threads` threads, and one step with one thread on the
+    workload's first `one_thread_grains` grains (a bounded sample: the whole
+    workload takes about a minute per step on one thread)."""
     import oracle
     o = oracle.Oracle(setup, params, threads=threads)
     o.step(1)  # warm-up
@@ -64,13 +70,24 @@ def cpu_baseline(setup, params, threads, timed_steps=2):
     o.step(timed_steps)
     dt = time.perf_counter() - t0
     o.close()
+    from perigrain_amd import setups
+    sub = setups.subset(setup, list(range(min(one_thread_grains, setup.n_grains))))
+    o1 = oracle.Oracle(sub, params, threads=1)
+    o1.step(1)
+    t0 = time.perf_counter()
+    o1.step(1)
+    dt1 = time.perf_counter() - t0
+    o1.close()
     return {
         "value": setup.n_nodes * timed_steps / dt,
         "unit": "node-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{timed_steps} steps after 1 warm-up step of the same 3D workload "
-                  f"({setup.n_nodes} nodes), {dt:.1f} s",
+                  f"({setup.n_nodes} nodes), {dt:.1f} s on {threads} threads",
+        "one_thread": {"value": sub.n_nodes / dt1, "unit": "node-steps/s", "cores": 1,
+                       "sample": f"1 step after 1 warm-up step of the workload's first "
+                                 f"{sub.n_grains} grains ({sub.n_nodes} nodes), {dt1:.1f} s"},
     }
 
 
@@ -255,6 +272,54 @@ def secondary_bench(threads, with_cpu):
     return out
 
 
+def dropin_bench(setup, params, steps=10000, modulo=1000):
+    """The drop-in executable (bin/simulate3d) on the headline workload with
+    a snapshot every `modulo` steps, against pure stepping of the same steps
+    through pg_step: the snapshots go out from a writer thread while the
+    steps go on, so the time loop should take about as long as pure
+    stepping.  Runs in a temporary directory laid out like the reference's
+    root, removed afterwards."""
+    import shutil
+    import subprocess
+    import tempfile
+    from perigrain_amd import Stepper, h5io
+    p = dataclasses.replace(params, timesteps=steps, modulo=modulo)
+    tmp = tempfile.mkdtemp(prefix="pg_dropin_")
+    try:
+        os.makedirs(os.path.join(tmp, "config"))
+        os.makedirs(os.path.join(tmp, "data", "hdf5"))
+        h5io.write_all_h5(os.path.join(tmp, "data", "hdf5", "all.h5"), setup)
+        keys = h5io.wall_speed_keys(setup.dim, p.wall_speed[:2 * setup.dim])
+        h5io.write_config(os.path.join(tmp, "config", "main.conf"), p, **keys)
+        with Stepper(setup, p, device=0) as g:
+            g.step(10)
+            g.sync()
+        with Stepper(setup, p, device=0) as g:
+            g.sync()
+            t0 = time.perf_counter()
+            g.step(steps)
+            g.sync()
+            pure = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = subprocess.run([h5io.simulate_exe(setup.dim)], cwd=tmp, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError("simulate3d failed: " + r.stdout[-2000:])
+        loop = [float(x) for x in h5io.read_dataset(
+            os.path.join(tmp, "output", "hdf5", "run_time.h5"), "run_time").ravel()]
+        out = os.path.join(tmp, "output", "hdf5")
+        written = sum(os.path.getsize(os.path.join(out, f)) for f in os.listdir(out))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    rec = {"steps": steps, "modulo": modulo, "snapshots": steps // modulo,
+           "pure_stepping_s": pure, "time_loop_s": sum(loop) / 1e3,
+           "time_loop_over_pure": sum(loop) / 1e3 / pure, "process_wall_s": wall,
+           "bytes_written": written}
+    print(f"bench: drop-in {rec}", file=sys.stderr, flush=True)
+    return rec
+
+
 def measured_traffic(workload, n_nodes, n_slots):
     """HBM bytes per step-kernel launch from the committed PMC summary, if
     it was measured on this exact workload; else None."""
@@ -282,6 +347,8 @@ def main():
                     help="Verlet skin factor and block size instead of the autotuner's choice")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 2D two-disk and bulk-settle workloads")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in executable's snapshot-overlap leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
@@ -351,6 +418,10 @@ def main():
     if not args.no_adjoint:
         adj = adjoint_bench()
 
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_bench(setup, params)
+
     threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
     others = None
     if not args.no_configs:
@@ -398,6 +469,7 @@ def main():
         "batched": batched,
         "adjoint": adj,
         "other_workloads": others,
+        "dropin": dropin,
     }
     print(json.dumps(line), flush=True)
 

@@ -261,13 +261,18 @@ int pg_stats(pg_handle *h, uint64_t *rebuilds, uint64_t *list_entries,
  * kernel (64, 128 or 256).  The contact list is rebuilt at once. */
 int pg_set_tuning(pg_handle *h, double skin_factor, int block);
 
-/* Autotuner (no reference counterpart): runs the next trial_steps steps
- * once per candidate configuration -- Verlet skin 0.25, 0.5 or 1.0 times
- * the contact radius, step-kernel blocks of 64, 128 or 256 threads -- timing
- * each on the host clock including neighbour-list rebuilds, and keeps the
- * fastest.  The trials are ordinary steps of the simulation (9*trial_steps
- * in total): skin and block size change which candidate pairs are listed and
- * how threads are grouped, never a result. */
+/* Autotuner (no reference counterpart): runs the next trial_steps steps once
+ * untimed (warm-up), then once per candidate configuration -- Verlet skin
+ * 0.25, 0.5 or 1.0 times the contact radius, step-kernel blocks of 64, 128
+ * or 256 threads -- timing each on the host clock including neighbour-list
+ * rebuilds, and keeps the fastest.  The trials are ordinary steps of the
+ * simulation (10*trial_steps in total): skin and block size change which
+ * candidate pairs are listed and how threads are grouped, never a result.
+ * If the environment variable PG_AUTOTUNE_CACHE names a file, a choice
+ * recorded there for the same workload (dimension, node, slot, grain,
+ * instance and contact-key counts) and device (architecture, CU count) is
+ * applied without trials (and no steps run), and a new choice is appended
+ * to it. */
 int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *block);
 
 /* Replay whole chunks of step launches (64 steps that do not include the

@@ -1757,6 +1757,43 @@ int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *bl
     if (!h || trial_steps == 0) return fail(PG_ERR_INVALID, "null handle or zero trial steps");
     static const double skins[] = {0.25, 0.5, 1.0};
     static const int blocks[] = {64, 128, 256};
+    // the workload and device the choice is valid for
+    hipDeviceProp_t prop{};
+    HIPCHK(hipGetDeviceProperties(&prop, h->device));
+    char key[512];
+    std::snprintf(key, sizeof key, "%s/%d dim=%d nodes=%u slots=%llu grains=%u insts=%u keys=%u",
+                  prop.gcnArchName, prop.multiProcessorCount, h->dim, h->N,
+                  (unsigned long long)h->E, h->G, h->I, h->CK);
+    const char *cache = std::getenv("PG_AUTOTUNE_CACHE");
+    if (cache && *cache) {
+      if (FILE *f = std::fopen(cache, "r")) {
+        char line[640];
+        double best_skin = 0.0;
+        int best_block = 0;
+        while (std::fgets(line, sizeof line, f)) {
+          const char *bar = std::strstr(line, " | ");
+          if (!bar || (size_t)(bar - line) != std::strlen(key) || std::strncmp(line, key, bar - line))
+            continue;
+          double sk = 0.0;
+          int bl = 0;
+          if (std::sscanf(bar + 3, "%lf %d", &sk, &bl) == 2) {
+            best_skin = sk;
+            best_block = bl;
+          }
+        }
+        std::fclose(f);
+        if (best_block) { // the last entry for this workload wins
+          RC(pg_set_tuning(h, best_skin, best_block));
+          if (skin_factor) *skin_factor = best_skin;
+          if (block) *block = best_block;
+          return PG_OK;
+        }
+      }
+    }
+    // one untimed run first, so the first candidate does not pay for cold
+    // caches and first-launch costs
+    RC(run_steps(h, trial_steps));
+    HIPCHK(hipStreamSynchronize(h->stream));
     double best = 1e300, best_skin = h->skin_factor;
     int best_block = h->block;
     for (double sk : skins)
@@ -1774,6 +1811,12 @@ int pg_autotune(pg_handle *h, uint32_t trial_steps, double *skin_factor, int *bl
         }
       }
     RC(pg_set_tuning(h, best_skin, best_block));
+    if (cache && *cache) {
+      if (FILE *f = std::fopen(cache, "a")) {
+        std::fprintf(f, "%s | %.17g %d\n", key, best_skin, best_block);
+        std::fclose(f);
+      }
+    }
     if (skin_factor) *skin_factor = best_skin;
     if (block) *block = best_block;
     return PG_OK;

@@ -273,10 +273,11 @@ __device__ __forceinline__ void block_box_init(BlockBox &bb, const StepArgs &A,
 }
 
 // wave_box_atomic's reduction, then the workgroup combination; all lanes of
-// every wavefront of the workgroup call it.
+// every wavefront of the workgroup call it, `calls` calls per workgroup in
+// all (the last one adds the workgroup's boxes to the global keys).
 template <int D>
 __device__ __forceinline__ void wave_box_block(unsigned long long *keys, uint32_t grp, bool valid,
-                                               const double *x, BlockBox &bb) {
+                                               const double *x, BlockBox &bb, uint32_t calls) {
   unsigned long long key[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) key[k] = dkey(x[k]);
@@ -327,7 +328,7 @@ __device__ __forceinline__ void wave_box_block(unsigned long long *keys, uint32_
   uint32_t prev = 0;
   if (lane0) prev = atomicAdd(&bb.done, 1u);
   prev = __builtin_amdgcn_readfirstlane(prev);
-  if (prev + 1 == blockDim.x / 64) {
+  if (prev + 1 == calls) {
     __threadfence_block();
     const uint32_t l = threadIdx.x & 63;
     if (l < BOX_SLOTS * 2 * D && bb.g0 != 0xFFFFFFFFu) {
@@ -709,7 +710,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
   }
 
   if (viol) atomicMin(A.violation, t + 1);
-  if (bb) wave_box_block<D>(A.bbox[(t + 1) % 3], g, valid, xn, *bb);
+  if (bb) wave_box_block<D>(A.bbox[(t + 1) % 3], g, valid, xn, *bb, blockDim.x / 64);
   else wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
 }
 
@@ -1121,10 +1122,24 @@ __device__ __forceinline__ bool contact_x(const StepArgs &A, uint32_t t, uint32_
 }
 
 // Per-instance box of contact nodes at step t (list build).
+// Workgroups of 1024 nodes combine their wavefronts' boxes in LDS first, so
+// an instance's keys see one global atomic per workgroup, not per wavefront.
 template <int D>
-__global__ __launch_bounds__(256) void k_inst_box(const StepArgs A, const uint32_t t,
-                                                  unsigned long long *ibox) {
+__global__ __launch_bounds__(1024) void k_inst_box(const StepArgs A, const uint32_t t,
+                                                   unsigned long long *ibox) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ BlockBox bb;
+  if (threadIdx.x < BOX_SLOTS * 2 * D) {
+    const uint32_t k = threadIdx.x % (2 * D);
+    bb.key[threadIdx.x / (2 * D)][k] = k < (uint32_t)D ? ~0ull : 0ull;
+  }
+  if (threadIdx.x == 0) {
+    // instance of the workgroup's first node: the smallest of its nodes'
+    const uint32_t f = blockIdx.x * blockDim.x;
+    bb.g0 = f < A.n_nodes ? A.grains[A.node_grain[f]].inst : 0xFFFFFFFFu;
+    bb.done = 0;
+  }
+  __syncthreads();
   bool valid = false;
   uint32_t inst = 0xFFFFFFFFu;
   double x[D];
@@ -1132,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_inst_box(const StepArgs A, const uint32
   for (int k = 0; k < D; ++k) x[k] = 0.0;
   if (i < A.n_nodes) valid = contact_x<D>(A, t, i, x, inst);
   if (!valid) inst = 0xFFFFFFFFu;
-  wave_box_atomic<D>(ibox, inst, valid, x);
+  wave_box_block<D>(ibox, inst, valid, x, bb, blockDim.x / 64);
 }
 
 template <int D>
@@ -1478,9 +1493,9 @@ hipError_t launch_fill_box(unsigned long long *p, size_t n, int dim, hipStream_t
 
 hipError_t launch_inst_box(int dim, const StepArgs &A, uint32_t t, unsigned long long *ibox,
                            hipStream_t s) {
-  const uint32_t nb = nblk(A.n_nodes, 256) > 0 ? nblk(A.n_nodes, 256) : 1;
-  if (dim == 2) hipLaunchKernelGGL(k_inst_box<2>, dim3(nb), dim3(256), 0, s, A, t, ibox);
-  else hipLaunchKernelGGL(k_inst_box<3>, dim3(nb), dim3(256), 0, s, A, t, ibox);
+  const uint32_t nb = nblk(A.n_nodes, 1024) > 0 ? nblk(A.n_nodes, 1024) : 1;
+  if (dim == 2) hipLaunchKernelGGL(k_inst_box<2>, dim3(nb), dim3(1024), 0, s, A, t, ibox);
+  else hipLaunchKernelGGL(k_inst_box<3>, dim3(nb), dim3(1024), 0, s, A, t, ibox);
   return hipGetLastError();
 }
 

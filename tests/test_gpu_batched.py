@@ -57,9 +57,34 @@ def test_autotune_is_result_neutral():
     with Stepper(s, p, device=0) as tuned, Stepper(s, p, device=0) as plain:
         skin, block = tuned.autotune(trial_steps=20)
         assert skin in (0.25, 0.5, 1.0) and block in (64, 128, 256)
-        assert tuned.t == 180
-        plain.step(180)
+        assert tuned.t == 200  # a warm-up run and nine trials
+        plain.step(200)
         _same(plain, tuned)
         tuned.step(150)
         plain.step(150)
         _same(plain, tuned)
+
+
+def test_autotune_choice_is_persisted_per_workload(tmp_path, monkeypatch):
+    # with PG_AUTOTUNE_CACHE set, a choice recorded for the same workload is
+    # applied without trials; another workload gets its own entry
+    cache = tmp_path / "autotune.txt"
+    monkeypatch.setenv("PG_AUTOTUNE_CACHE", str(cache))
+    s, p = _fast_settle()
+    with Stepper(s, p, device=0) as a:
+        choice = a.autotune(trial_steps=8)
+        assert a.t == 80
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and "dim=2" in lines[0]
+    # the recorded choice is reused by a new handle of the same workload: no steps run
+    cache.write_text(lines[0].split(" | ")[0] + " | 1 64\n")
+    with Stepper(s, p, device=0) as b:
+        assert b.autotune(trial_steps=8) == (1.0, 64)
+        assert b.t == 0
+    s2, p2 = setups.two_disk_collision(seed=1)
+    with Stepper(s2, p2, device=0) as c:
+        c.autotune(trial_steps=4)
+        assert c.t == 40
+    assert len(cache.read_text().splitlines()) == 2
+    assert choice[0] in (0.25, 0.5, 1.0)
+
