@@ -915,7 +915,8 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
 //
 // k_wall_compact: the keys whose node touched a wall, compacted in key
 // order, in parallel over chunks of REACT_CHUNK keys: chunk b of recorded
-// step y writes its hit keys, ascending, to hits[y][b*REACT_CHUNK ...] and
+// step y writes its hits, ascending, to hits[y][b*REACT_CHUNK ...] -- each
+// the key's offset in the chunk with the side mask in bits 16-23 -- and
 // their number to chunk_cnt[y][b].  One block per (chunk, step).
 constexpr uint32_t REACT_CHUNK = 4096;           // keys per chunk: 16 per thread
 constexpr uint32_t REACT_PIECE = 256;            // hits staged in LDS at a time
@@ -951,26 +952,27 @@ __global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q)
-    if ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) hits[pos++] = k0 + q;
+    if (const uint32_t m = (w[q >> 2] >> (8 * (q & 3))) & 0xFFu) // key within the chunk | sides
+      hits[pos++] = (tid * 16 + q) | (m << 16);
   if (tid == 0) R.chunk_cnt[blockIdx.y * R.cstride + blockIdx.x] = total;
 }
 
 // k_reaction: one block per recorded step.  The compacted hits are taken in
-// pieces of REACT_PIECE keys.  The whole block stages a piece: each hit's
-// side mask (the flag byte: bit s for a touched side s) and the four terms
-// of every side it touched -- normal, damping, friction along the side's two
-// tangents -- go into one list per side, in key order (a packed block scan
-// of the per-side counts gives the positions).  Then one thread per
+// pieces of REACT_PIECE.  The whole block stages a piece: the four terms of
+// every side each hit touched -- normal, damping, friction along the side's
+// two tangents -- go into one list per side, in key order (a packed block
+// scan of the per-side counts gives the positions).  Then one thread per
 // (instance, side, component) subtracts its side's terms in key order:
 // terms 0 and 1 along the side's direction, term 2 or 3 across it.  A node
 // that did not touch a side adds nothing to it (in the reference that side
 // is not updated; here it is not visited).  Running sums live in the output
-// between pieces.
+// between pieces.  Each thread loads its hit of the next piece (key, sides,
+// terms) before the current piece is summed, so those loads overlap the
+// sums.
 template <int D>
 __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
   const uint32_t slot = (R.first + blockIdx.y) % R.slots;
   const double *wterm = R.wterm + slot * R.wstride;
-  const uint8_t *wflag = R.wflag + slot * R.fstride;
   const uint32_t *hits = R.hits + blockIdx.y * R.hstride;
   const uint32_t *ccnt = R.chunk_cnt + blockIdx.y * R.cstride;
   double *reaction = R.reaction + blockIdx.y * R.rstride;
@@ -1005,11 +1007,13 @@ __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
     s_off[tid] = excl;
     if (tid == 0) s_off[256] = total;
     __syncthreads();
-    for (uint32_t p0 = 0; p0 < total; p0 += REACT_PIECE) {
-      const uint32_t np = min(REACT_PIECE, total - p0);
-      uint32_t key = 0, mask = 0;
-      double2 t[NS][2];
-      if (tid < np) {
+    // this thread's hit of the piece starting at p0: key, side mask, terms
+    uint32_t key = 0, mask = 0;
+    double2 tt[NS][2];
+    auto fetch = [&](uint32_t p0) {
+      key = 0;
+      mask = 0;
+      if (p0 + tid < total) {
         // chunk of hit e: the last chunk whose offset is <= e
         const uint32_t e = p0 + tid;
         uint32_t lo = 0, hi = 256;
@@ -1018,16 +1022,20 @@ __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
           if (s_off[mid] <= e) lo = mid;
           else hi = mid;
         }
-        key = hits[(size_t)(g0 + lo) * REACT_CHUNK + (e - s_off[lo])];
-        mask = wflag[key];
+        const uint32_t h = hits[(size_t)(g0 + lo) * REACT_CHUNK + (e - s_off[lo])];
+        key = (g0 + lo) * REACT_CHUNK + (h & 0xFFFFu);
+        mask = h >> 16;
         const double2 *src = reinterpret_cast<const double2 *>(wterm + (size_t)key * 24);
 #pragma unroll
         for (int sd = 0; sd < NS; ++sd)
           if ((mask >> sd) & 1u) {
-            t[sd][0] = src[2 * sd];
-            t[sd][1] = src[2 * sd + 1];
+            tt[sd][0] = src[2 * sd];
+            tt[sd][1] = src[2 * sd + 1];
           }
       }
+    };
+    fetch(0);
+    for (uint32_t p0 = 0; p0 < total; p0 += REACT_PIECE) {
       // positions in the per-side lists: a block scan of the per-side
       // counts packed 9 bits per side (a piece has at most 256 hits)
       unsigned long long pk = 0;
@@ -1052,10 +1060,11 @@ __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
         if ((mask >> sd) & 1u) {
           const uint32_t ps = (uint32_t)(pos >> (9 * sd)) & 511u;
           s_key[sd][ps] = key;
-          s_side[sd][ps][0] = t[sd][0];
-          s_side[sd][ps][1] = t[sd][1];
+          s_side[sd][ps][0] = tt[sd][0];
+          s_side[sd][ps][1] = tt[sd][1];
         }
       __syncthreads();
+      fetch(p0 + REACT_PIECE); // the next piece's loads overlap the sums below
       for (uint32_t a = tid; a < n_assign; a += blockDim.x) {
         const uint32_t inst = a / PER_INST, sd = (a / D) % NS, cp = a % D;
         const uint32_t nside = (uint32_t)(tot >> (9 * sd)) & 511u;
