@@ -153,6 +153,19 @@ int pg_check_arith(int device, uint64_t n, const double *a, const double *b, dou
 int pg_check_bond(int device, int dim, int check, uint64_t n, const double *xi, const double *eta,
                   double *s, double *e);
 
+/* Diagnostics (no reference counterpart): the wall reaction's ordered
+ * subtraction (contact.h:339,374,411-412,428 accumulate one wall side's
+ * reaction this way).  Chain c holds entries entry_off[c] .. entry_off[c+1]-1
+ * of `entries` ([entry_off[nchains]][4] doubles, at most 2^26) and its terms
+ * are terms 0 and 1 of every entry (two[c] != 0) or term 2; the result is
+ * s0[c] minus its terms one at a time, in order.  wave[c] is that result as
+ * the reaction kernel evaluates a long chain (one wavefront, chunks from
+ * guessed start values corrected until exact); seq[c] is one device
+ * thread's plain loop. */
+int pg_check_ordered_sum(int device, uint32_t nchains, const uint64_t *entry_off,
+                         const int32_t *two, const double *s0, const double *entries,
+                         double *wave, double *seq);
+
 /* Create a stepper for one simulation on HIP device `device`.
  * Replaces the state set-up of run_timeloop (timeloop.h:504-557; gen_xi at
  * particle2.h:236-252 and conn2NArr at load_hdf5.h:28-38) without the file

@@ -210,6 +210,8 @@ def lib():
         "pg_check_arith": [ctypes.c_int, ctypes.c_uint64, _dp, _dp, _dp, _dp],
         "pg_check_bond": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _dp, _dp,
                           _dp, _dp],
+        "pg_check_ordered_sum": [ctypes.c_int, ctypes.c_uint32, _u64p, ctypes.POINTER(ctypes.c_int32),
+                                 _dp, _dp, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -2131,6 +2131,47 @@ extern "C" int pg_check_bond(int device, int dim, int check, uint64_t n, const d
   });
 }
 
+extern "C" int pg_check_ordered_sum(int device, uint32_t nchains, const uint64_t *entry_off,
+                                    const int32_t *two, const double *s0, const double *entries,
+                                    double *wave, double *seq) {
+  return guarded([&]() -> int {
+    if (!entry_off || !two || !s0 || !entries || !wave || !seq)
+      return fail(PG_ERR_INVALID, "null argument");
+    if (entry_off[0] != 0) return fail(PG_ERR_INVALID, "entry_off[0] must be 0");
+    for (uint32_t c = 0; c < nchains; ++c)
+      if (entry_off[c + 1] < entry_off[c]) return fail(PG_ERR_INVALID, "entry_off decreases");
+    const uint64_t ne = entry_off[nchains];
+    if (ne > (1ull << 26)) return fail(PG_ERR_INVALID, "more than 2^26 entries");
+    if (nchains == 0) return PG_OK;
+    HIPCHK(hipSetDevice(device));
+    // device layout: entries (+1 zero entry), s0, wave, seq, offsets, flags
+    const size_t nd = (ne + 1) * 4 + 3 * (size_t)nchains;
+    const size_t bytes = nd * sizeof(double) + (nchains + 1) * sizeof(uint64_t) +
+                         nchains * sizeof(int32_t);
+    char *d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    double *dent = reinterpret_cast<double *>(d), *ds0 = dent + (ne + 1) * 4;
+    double *dwave = ds0 + nchains, *dseq = dwave + nchains;
+    uint64_t *doff = reinterpret_cast<uint64_t *>(dseq + nchains);
+    int32_t *dtwo = reinterpret_cast<int32_t *>(doff + nchains + 1);
+    hipError_t err = hipMemcpy(dent, entries, ne * 4 * sizeof(double), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(dent + ne * 4, 0, 4 * sizeof(double));
+    if (err == hipSuccess) err = hipMemcpy(ds0, s0, nchains * sizeof(double), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = hipMemcpy(doff, entry_off, (nchains + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(dtwo, two, nchains * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = launch_chain_check(nchains, dent, doff, dtwo, ds0, (uint32_t)ne, dwave, dseq, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(wave, dwave, nchains * sizeof(double), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(seq, dseq, nchains * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (err != hipSuccess)
+      return fail(PG_ERR_DEVICE, std::string("pg_check_ordered_sum: ") + hipGetErrorString(err));
+    return PG_OK;
+  });
+}
+
 extern "C" int pg_check_arith(int device, uint64_t n, const double *a, const double *b, double *q,
                               double *r) {
   return guarded([&]() -> int {

@@ -73,6 +73,9 @@ hipError_t launch_bond_geom(int dim, bool check, uint64_t n, const double *xi, c
                             double *s, double *e, hipStream_t st);
 hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q, double *r,
                         hipStream_t s);
+hipError_t launch_chain_check(uint32_t nchains, const double *ent, const uint64_t *eoff,
+                              const int32_t *two, const double *s0, uint32_t zero, double *wave,
+                              double *seq, hipStream_t s);
 // hipcub exclusive sum; call with tmp=nullptr to size *tmp_bytes.
 hipError_t exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp,
                               size_t *tmp_bytes, hipStream_t s);

@@ -919,7 +919,6 @@ __global__ void k_grain_mean(const StepArgs A, const uint32_t t_arg, double *gme
 // the key's offset in the chunk with the side mask in bits 16-23 -- and
 // their number to chunk_cnt[y][b].  One block per (chunk, step).
 constexpr uint32_t REACT_CHUNK = 4096;           // keys per chunk: 16 per thread
-constexpr uint32_t REACT_PIECE = 256;            // hits staged in LDS at a time
 static_assert(REACT_CHUNK == (uint32_t)kWflagSpare, "flag loads run one chunk past the keys");
 
 __global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
@@ -958,19 +957,183 @@ __global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
 }
 
 // k_reaction: one block per recorded step.  The compacted hits are taken in
-// pieces of REACT_PIECE.  The whole block stages a piece: the four terms of
-// every side each hit touched -- normal, damping, friction along the side's
-// two tangents -- go into one list per side, in key order (a packed block
-// scan of the per-side counts gives the positions).  Then one thread per
-// (instance, side, component) subtracts its side's terms in key order:
-// terms 0 and 1 along the side's direction, term 2 or 3 across it.  A node
-// that did not touch a side adds nothing to it (in the reference that side
-// is not updated; here it is not visited).  Running sums live in the output
-// between pieces.  Each thread loads its hit of the next piece (key, sides,
-// terms) before the current piece is summed, so those loads overlap the
-// sums.
+// pieces of REACT_PIECE, one per thread.  The block stages a piece: the four
+// terms of every side each hit touched -- normal, damping, friction along the
+// side's two tangents -- go into one list per side, in key order (ballots
+// give the positions).  Each (instance,
+// side, component) is a chain of subtractions in key order: terms 0 and 1 of
+// every entry along the side's direction, term 2 or 3 across it.  A node that
+// did not touch a side adds nothing to it (in the reference that side is not
+// updated; here it is not visited).  Running sums stay in LDS between pieces
+// (the first REACT_LDS_SUMS of them; the rest in the output).  Each thread
+// loads its hit of the next piece (key, sides, terms) before the current
+// piece is summed, so those loads overlap the sums.
+//
+// A chain of at most REACT_SEQ_MAX terms is summed by one thread in order.
+// A longer one -- one wall pressed by many nodes -- would be a long chain of
+// dependent FP64 subtractions, so a wavefront evaluates it in 64 chunks at
+// once (react_chain_wave), each from a guessed start value, and corrects the
+// guesses until every chunk starts exactly where the one before it ends:
+// then every chunk repeated the sequential subtractions from the sequential
+// start value, and the last chunk's end is the sequential result, bit for
+// bit.
+constexpr uint32_t REACT_PIECE = 512;      // hits staged in LDS at a time, one per thread
+constexpr uint32_t REACT_THREADS = REACT_PIECE;
+constexpr uint32_t REACT_SEQ_MAX = 64;     // longest chain summed by one thread
+constexpr uint32_t REACT_LONG_MAX = 192;   // longer chains of a piece handed to wavefronts
+constexpr uint32_t REACT_LDS_SUMS = 18 * 64; // running sums held in LDS
+constexpr int REACT_BATCH = 8;             // terms read ahead of a chain
+
+template <int NS> struct ReactHit {
+  uint32_t key, mask; // mask bit s: the node touched side s
+  double2 t[NS][2];   // terms (0, 1), (2, 3) of each side it touched
+};
+
+// Hit e of the chunk group that starts at chunk g0 (s_off: the group's chunk
+// offsets in the list of hits, s_off[ngroup] = total): the last chunk whose
+// offset is <= e holds it.
+template <int NS>
+__device__ __forceinline__ void react_fetch(const uint32_t *hits, const double *wterm,
+                                            const uint32_t *s_off, uint32_t ngroup, uint32_t g0,
+                                            uint32_t total, uint32_t e, ReactHit<NS> &h) {
+  h.key = 0;
+  h.mask = 0;
+  if (e >= total) return;
+  uint32_t lo = 0, hi = ngroup;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t w = hits[(size_t)(g0 + lo) * REACT_CHUNK + (e - s_off[lo])];
+  h.key = (g0 + lo) * REACT_CHUNK + (w & 0xFFFFu);
+  h.mask = w >> 16;
+  const double2 *src = reinterpret_cast<const double2 *>(wterm + (size_t)h.key * 24);
+#pragma unroll
+  for (int sd = 0; sd < NS; ++sd)
+    if ((h.mask >> sd) & 1u) {
+      h.t[sd][0] = src[2 * sd];
+      h.t[sd][1] = src[2 * sd + 1];
+    }
+}
+
+// Cross-lane moves of a double by DPP (two 32-bit halves): lanes whose
+// source lies outside the pattern, or whose row ROW_MASK leaves out, get +0.0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143,
+              DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+// lane l gets lane l+1's x (lane 63: +0.0), and lane l-1's (lane 0: +0.0)
+__device__ __forceinline__ double lane_next(double x) { return dpp_d<DPP_WAVE_SHL1, 0xF>(x); }
+__device__ __forceinline__ double lane_prev(double x) { return dpp_d<DPP_WAVE_SHR1, 0xF>(x); }
+
+// Exclusive prefix sum over the wavefront's lanes, in tree order (all lanes
+// active).  Used only for guesses: the results are checked, not trusted.
+__device__ __forceinline__ double wave_excl_sum(double v) {
+  double x = v;
+  x = x + dpp_d<DPP_ROW_SHR1, 0xF>(x); // sums within rows of 16 lanes
+  x = x + dpp_d<DPP_ROW_SHR2, 0xF>(x);
+  x = x + dpp_d<DPP_ROW_SHR4, 0xF>(x);
+  x = x + dpp_d<DPP_ROW_SHR8, 0xF>(x);
+  x = x + dpp_d<DPP_ROW_BCAST15, 0xA>(x); // row 0's total into row 1, row 2's into row 3
+  x = x + dpp_d<DPP_ROW_BCAST31, 0xC>(x); // lane 31's total into rows 2 and 3
+  return lane_prev(x);
+}
+__device__ __forceinline__ double read_lane_d(double x, uint32_t l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, (int)l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), (int)l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
+// The terms of one chain are base[idx(m)], m = 0 .. n-1: entry lo + m/2,
+// terms 0 and 1, when the component is the side's direction (two); entry
+// lo + m, term o1, otherwise.  Indices past `end` read the all-zero entry
+// (x - (+0.0) == x for every x, zeros and NaNs included), so reads run in
+// batches with no select in the chain.
+struct ChainTerms {
+  const double *base;
+  uint32_t lo, o1, zero;
+  bool two;
+  __device__ __forceinline__ uint32_t idx(uint32_t m, uint32_t end) const {
+    if (m >= end) return zero * 4;
+    return two ? (lo + (m >> 1)) * 4 + (m & 1u) : (lo + m) * 4 + o1;
+  }
+  // x minus terms m0 .. m1-1, in order
+  __device__ __forceinline__ double run(double x, uint32_t m0, uint32_t m1) const {
+    for (uint32_t m = m0; m < m1; m += REACT_BATCH) {
+      double v[REACT_BATCH];
+#pragma unroll
+      for (int k = 0; k < REACT_BATCH; ++k) v[k] = base[idx(m + (uint32_t)k, m1)];
+#pragma unroll
+      for (int k = 0; k < REACT_BATCH; ++k) x = x - v[k];
+    }
+    return x;
+  }
+};
+
+// s0 minus the n terms of a chain in order, by one wavefront (all lanes call
+// it; every lane returns the result).  Lane c takes chunk c, terms
+// [c*L, (c+1)*L); chunks 0 .. C-1 are not empty.  Each round evaluates every
+// chunk from its guess g_c; g_0 = s0 always.  If every g_{c+1} equals chunk
+// c's end bit for bit, all chunks started from the sequential values (by
+// induction from g_0) and chunk C-1's end is the answer.  Otherwise, with cs
+// the first chunk whose end differs from the next guess, chunks 0 .. cs
+// started exactly, so g_{cs+1} becomes chunk cs's end -- one more exact start
+// per round, at most C + 1 rounds -- and the later guesses move by the
+// differences summed from cs on, which makes them exact as well whenever a
+// chunk's end shifts by its start's shift (the usual case: no rounding to a
+// coarser binade inside the chunk).  The first guesses are s0 minus the
+// chunk sums of the chunks before.  Chunks of up to REACT_REG_TERMS terms
+// (every chain of a reaction piece) stay in registers across the rounds.
+constexpr uint32_t REACT_REG_TERMS = 16;
+__device__ __forceinline__ double react_chain_wave(const ChainTerms &ct, uint32_t n, double s0,
+                                                   uint32_t lane) {
+  if (n == 0) return s0;
+  const uint32_t L = (n + 63) / 64, C = (n + L - 1) / L;
+  const uint32_t m0 = min(lane * L, n), m1 = min(m0 + L, n);
+  const bool regs = L <= REACT_REG_TERMS; // the same on every lane
+  double v[REACT_REG_TERMS];
+  if (regs) {
+#pragma unroll
+    for (uint32_t k = 0; k < REACT_REG_TERMS; ++k) v[k] = ct.base[ct.idx(m0 + k, m1)];
+  }
+  auto chunk = [&](double x) -> double {
+    if (regs) {
+#pragma unroll
+      for (uint32_t k = 0; k < REACT_REG_TERMS; ++k) x = x - v[k];
+      return x;
+    }
+    return ct.run(x, m0, m1);
+  };
+  const double part = chunk(0.0);
+  const double pre = wave_excl_sum(part);
+  double g = (lane == 0) ? s0 : s0 + pre;
+  double end = g;
+  for (uint32_t round = 0; round <= C; ++round) {
+    end = chunk(g);
+    const double gn = lane_next(g);
+    const bool bad =
+        lane + 1 < C && __double_as_longlong(gn) != __double_as_longlong(end);
+    const unsigned long long mb = __ballot(bad);
+    if (mb == 0) break;
+    const uint32_t cs = (uint32_t)__builtin_ctzll(mb);
+    const double dx = wave_excl_sum((lane >= cs && lane + 1 < C) ? end - gn : 0.0);
+    const double prev_end = lane_prev(end);
+    if (lane == cs + 1) g = prev_end;
+    else if (lane > cs + 1 && lane < C) g = g + dx;
+  }
+  return read_lane_d(end, C - 1);
+}
+
 template <int D>
-__global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
+__global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
   const uint32_t slot = (R.first + blockIdx.y) % R.slots;
   const double *wterm = R.wterm + slot * R.wstride;
   const uint32_t *hits = R.hits + blockIdx.y * R.hstride;
@@ -978,16 +1141,42 @@ __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
   double *reaction = R.reaction + blockIdx.y * R.rstride;
   constexpr int NS = (D == 2) ? 4 : 6;
   constexpr uint32_t PER_INST = NS * D;
-  __shared__ uint32_t s_off[257];
-  __shared__ uint32_t wtot[4];
-  __shared__ unsigned long long wtot64[4];
+  constexpr uint32_t NW = REACT_THREADS / 64;
+  constexpr uint32_t ZERO = REACT_PIECE; // index of each side list's all-zero entry
+  __shared__ uint32_t s_off[REACT_THREADS + 1];
+  __shared__ uint32_t wtot[NW];
+  __shared__ uint32_t s_wcnt[NW][NS]; // hits of a piece per wavefront and side
   __shared__ uint32_t s_key[NS][REACT_PIECE];
-  __shared__ double2 s_side[NS][REACT_PIECE][2]; // terms (0, 1), (2, 3) of one side
+  __shared__ double2 s_side[NS][REACT_PIECE + 1][2]; // terms (0, 1), (2, 3) of one side
+  __shared__ double s_acc[REACT_LDS_SUMS];
+  __shared__ uint32_t s_long[REACT_LONG_MAX][3]; // assignment, first entry, entry end
+  __shared__ uint32_t s_nlong;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t n_assign = R.n_insts * PER_INST;
-  for (uint32_t a = tid; a < n_assign; a += blockDim.x)
-    reaction[(size_t)(a / PER_INST) * 18 + ((a / D) % NS) * 3 + a % D] = 0.0;
-  for (uint32_t g0 = 0; g0 < R.nchunks; g0 += 256) {
+  auto out_at = [&](uint32_t a) {
+    return reaction + (size_t)(a / PER_INST) * 18 + ((a / D) % NS) * 3 + a % D;
+  };
+  // the terms of assignment a's chain among entries [lo, e1) of its side
+  auto chain_of = [&](uint32_t a, uint32_t lo) {
+    const uint32_t sd = (a / D) % NS, cp = a % D;
+    const int dir = (D == 2) ? ((sd < 2) ? 0 : 1) : (int)(sd % 3);
+    ChainTerms ct;
+    ct.base = reinterpret_cast<const double *>(&s_side[sd][0][0]);
+    ct.lo = lo;
+    ct.two = (int)cp == dir;
+    ct.o1 = ct.two ? 0u : (((int)cp == (dir + 1) % D) ? 2u : 3u);
+    ct.zero = ZERO;
+    return ct;
+  };
+  // a thread's short-chain sums are the assignments a = tid (mod
+  // REACT_THREADS): it alone clears, updates and writes them; a long chain's
+  // sum is read and written by a wavefront between two barriers
+  for (uint32_t a = tid; a < n_assign; a += REACT_THREADS) {
+    if (a < REACT_LDS_SUMS) s_acc[a] = 0.0;
+    else *out_at(a) = 0.0;
+  }
+  if (tid < 2 * NS) s_side[tid >> 1][ZERO][tid & 1] = make_double2(0.0, 0.0);
+  for (uint32_t g0 = 0; g0 < R.nchunks; g0 += REACT_THREADS) {
     // offsets of this group's chunks in the key-ordered list of hits
     const uint32_t c = (g0 + tid < R.nchunks) ? ccnt[g0 + tid] : 0u;
     uint32_t v = c;
@@ -1000,116 +1189,121 @@ __global__ __launch_bounds__(256) void k_reaction(ReactionArgs R) {
     __syncthreads();
     uint32_t excl = v - c, total = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
+    for (uint32_t q = 0; q < NW; ++q) {
       if (q < wave) excl += wtot[q];
       total += wtot[q];
     }
     s_off[tid] = excl;
-    if (tid == 0) s_off[256] = total;
+    if (tid == 0) s_off[REACT_THREADS] = total;
     __syncthreads();
-    // this thread's hit of the piece starting at p0: key, side mask, terms
-    uint32_t key = 0, mask = 0;
-    double2 tt[NS][2];
-    auto fetch = [&](uint32_t p0) {
-      key = 0;
-      mask = 0;
-      if (p0 + tid < total) {
-        // chunk of hit e: the last chunk whose offset is <= e
-        const uint32_t e = p0 + tid;
-        uint32_t lo = 0, hi = 256;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_off[mid] <= e) lo = mid;
-          else hi = mid;
-        }
-        const uint32_t h = hits[(size_t)(g0 + lo) * REACT_CHUNK + (e - s_off[lo])];
-        key = (g0 + lo) * REACT_CHUNK + (h & 0xFFFFu);
-        mask = h >> 16;
-        const double2 *src = reinterpret_cast<const double2 *>(wterm + (size_t)key * 24);
-#pragma unroll
-        for (int sd = 0; sd < NS; ++sd)
-          if ((mask >> sd) & 1u) {
-            tt[sd][0] = src[2 * sd];
-            tt[sd][1] = src[2 * sd + 1];
-          }
-      }
-    };
-    fetch(0);
+    ReactHit<NS> h;
+    react_fetch<NS>(hits, wterm, s_off, REACT_THREADS, g0, total, tid, h);
     for (uint32_t p0 = 0; p0 < total; p0 += REACT_PIECE) {
-      // positions in the per-side lists: a block scan of the per-side
-      // counts packed 9 bits per side (a piece has at most 256 hits)
-      unsigned long long pk = 0;
+      // positions in the per-side lists: per wavefront, the hits of lower
+      // lanes (ballot and bit count); across wavefronts, their totals
+      uint32_t wpos[NS];
 #pragma unroll
-      for (int sd = 0; sd < NS; ++sd) pk |= (unsigned long long)((mask >> sd) & 1u) << (9 * sd);
-      unsigned long long w = pk;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const unsigned long long u = __shfl_up(w, off, 64);
-        if (lane >= (uint32_t)off) w += u;
+      for (int sd = 0; sd < NS; ++sd) {
+        const unsigned long long b = __ballot((h.mask >> sd) & 1u);
+        wpos[sd] = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        if (lane == 0) s_wcnt[wave][sd] = (uint32_t)__popcll(b);
       }
-      if (lane == 63) wtot64[wave] = w;
+      if (tid == 0) s_nlong = 0;
       __syncthreads();
-      unsigned long long pos = w - pk, tot = 0;
-#pragma unroll
-      for (uint32_t q = 0; q < 4; ++q) {
-        if (q < wave) pos += wtot64[q];
-        tot += wtot64[q];
-      }
 #pragma unroll
       for (int sd = 0; sd < NS; ++sd)
-        if ((mask >> sd) & 1u) {
-          const uint32_t ps = (uint32_t)(pos >> (9 * sd)) & 511u;
-          s_key[sd][ps] = key;
-          s_side[sd][ps][0] = tt[sd][0];
-          s_side[sd][ps][1] = tt[sd][1];
+        if ((h.mask >> sd) & 1u) {
+          uint32_t ps = wpos[sd];
+          for (uint32_t q = 0; q < wave; ++q) ps += s_wcnt[q][sd];
+          s_key[sd][ps] = h.key;
+          s_side[sd][ps][0] = h.t[sd][0];
+          s_side[sd][ps][1] = h.t[sd][1];
         }
       __syncthreads();
-      fetch(p0 + REACT_PIECE); // the next piece's loads overlap the sums below
-      for (uint32_t a = tid; a < n_assign; a += blockDim.x) {
-        const uint32_t inst = a / PER_INST, sd = (a / D) % NS, cp = a % D;
-        const uint32_t nside = (uint32_t)(tot >> (9 * sd)) & 511u;
-        const int dir = (D == 2) ? ((sd < 2) ? 0 : 1) : (int)(sd % 3);
-        const int o1 = ((int)cp == dir) ? 0 : (((int)cp == (dir + 1) % D) ? 2 : 3);
-        const bool two = ((int)cp == dir);
+      // the next piece's loads overlap the sums below
+      react_fetch<NS>(hits, wterm, s_off, REACT_THREADS, g0, total, p0 + REACT_PIECE + tid, h);
+      for (uint32_t a = tid; a < n_assign; a += REACT_THREADS) {
+        const uint32_t inst = a / PER_INST, sd = (a / D) % NS;
+        uint32_t nside = 0;
+        for (uint32_t q = 0; q < NW; ++q) nside += s_wcnt[q][sd];
         // this instance's entries of the side's list (keys ascend, instances
         // own consecutive key ranges)
-        const uint32_t kb = R.insts[inst].ckey_begin, ke = R.insts[inst].ckey_end;
-        uint32_t lo = 0, hi = nside;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_key[sd][mid] < kb) lo = mid + 1;
-          else hi = mid;
-        }
-        uint32_t e1 = lo, e2 = nside;
-        while (e1 < e2) {
-          const uint32_t mid = (e1 + e2) >> 1;
-          if (s_key[sd][mid] < ke) e1 = mid + 1;
-          else e2 = mid;
+        uint32_t lo = 0, e1 = nside;
+        if (R.n_insts > 1) {
+          const uint32_t kb = R.insts[inst].ckey_begin, ke = R.insts[inst].ckey_end;
+          uint32_t hi = nside;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_key[sd][mid] < kb) lo = mid + 1;
+            else hi = mid;
+          }
+          e1 = lo;
+          uint32_t e2 = nside;
+          while (e1 < e2) {
+            const uint32_t mid = (e1 + e2) >> 1;
+            if (s_key[sd][mid] < ke) e1 = mid + 1;
+            else e2 = mid;
+          }
         }
         if (lo == e1) continue;
-        const double *base = reinterpret_cast<const double *>(&s_side[sd][0][0]);
-        double *out = reaction + (size_t)inst * 18 + sd * 3 + cp;
-        double acc = *out;
-        // eight entries' terms are read before they are subtracted in order
-        for (uint32_t e = lo; e < e1; e += 8) {
-          double a1[8], a2[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const uint32_t q = (e + k < e1) ? e + k : e;
-            a1[k] = base[q * 4 + o1];
-            a2[k] = base[q * 4 + 1];
+        const ChainTerms ct = chain_of(a, lo);
+        const uint32_t n = (e1 - lo) * (ct.two ? 2u : 1u);
+        if (n > REACT_SEQ_MAX) {
+          const uint32_t q = atomicAdd(&s_nlong, 1u);
+          if (q < REACT_LONG_MAX) {
+            s_long[q][0] = a;
+            s_long[q][1] = lo;
+            s_long[q][2] = e1;
+            continue;
           }
-#pragma unroll
-          for (int k = 0; k < 8; ++k)
-            if (e + k < e1) {
-              acc = acc - a1[k];
-              if (two) acc = acc - a2[k];
-            }
         }
-        *out = acc;
+        double acc = (a < REACT_LDS_SUMS) ? s_acc[a] : *out_at(a);
+        acc = ct.run(acc, 0, n);
+        if (a < REACT_LDS_SUMS) s_acc[a] = acc;
+        else *out_at(a) = acc;
+      }
+      __syncthreads();
+      const uint32_t nlong = min(s_nlong, REACT_LONG_MAX);
+      for (uint32_t q = wave; q < nlong; q += NW) {
+        const uint32_t a = s_long[q][0], lo = s_long[q][1], e1 = s_long[q][2];
+        const ChainTerms ct = chain_of(a, lo);
+        const uint32_t n = (e1 - lo) * (ct.two ? 2u : 1u);
+        const double s0 = (a < REACT_LDS_SUMS) ? s_acc[a] : *out_at(a);
+        const double r = react_chain_wave(ct, n, s0, lane);
+        if (lane == 0) {
+          if (a < REACT_LDS_SUMS) s_acc[a] = r;
+          else *out_at(a) = r;
+        }
       }
       __syncthreads();
     }
+  }
+  for (uint32_t a = tid; a < n_assign && a < REACT_LDS_SUMS; a += REACT_THREADS)
+    *out_at(a) = s_acc[a];
+}
+
+// Diagnostics: the wall reaction's ordered subtraction on given chains, so
+// tests can compare it with the host's sequential loop.  Chain c holds
+// entries [eoff[c], eoff[c+1]) of `ent` (4 doubles each; entry `zero` is all
+// zero) and takes terms 0 and 1 of every entry (two[c] != 0) or term 2.  One
+// wavefront per chain writes react_chain_wave's result to wave[c] and one
+// lane's plain loop to seq[c].
+__global__ __launch_bounds__(64) void k_chain_check(const double *ent, const uint64_t *eoff,
+                                                    const int32_t *two, const double *s0,
+                                                    uint32_t zero, double *wave, double *seq) {
+  const uint32_t c = blockIdx.x, lane = threadIdx.x;
+  ChainTerms ct;
+  ct.base = ent;
+  ct.lo = (uint32_t)eoff[c];
+  ct.two = two[c] != 0;
+  ct.o1 = ct.two ? 0u : 2u;
+  ct.zero = zero;
+  const uint32_t n = (uint32_t)(eoff[c + 1] - eoff[c]) * (ct.two ? 2u : 1u);
+  const double r = react_chain_wave(ct, n, s0[c], lane);
+  if (lane == 0) {
+    wave[c] = r;
+    seq[c] = ct.run(s0[c], 0, n);
   }
 }
 
@@ -1488,8 +1682,8 @@ hipError_t launch_reactions(int dim, const ReactionArgs &R, uint32_t nsteps, hip
   if (nsteps == 0) return hipSuccess;
   if (R.nchunks > 0)
     hipLaunchKernelGGL(k_wall_compact, dim3(R.nchunks, nsteps), dim3(256), 0, s, R);
-  if (dim == 2) hipLaunchKernelGGL(k_reaction<2>, dim3(1, nsteps), dim3(256), 0, s, R);
-  else hipLaunchKernelGGL(k_reaction<3>, dim3(1, nsteps), dim3(256), 0, s, R);
+  if (dim == 2) hipLaunchKernelGGL(k_reaction<2>, dim3(1, nsteps), dim3(REACT_THREADS), 0, s, R);
+  else hipLaunchKernelGGL(k_reaction<3>, dim3(1, nsteps), dim3(REACT_THREADS), 0, s, R);
   return hipGetLastError();
 }
 
@@ -1571,6 +1765,15 @@ hipError_t launch_arith(uint64_t n, const double *a, const double *b, double *q,
                         hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_arith, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, a, b, q, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain_check(uint32_t nchains, const double *ent, const uint64_t *eoff,
+                              const int32_t *two, const double *s0, uint32_t zero, double *wave,
+                              double *seq, hipStream_t s) {
+  if (nchains == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chain_check, dim3(nchains), dim3(64), 0, s, ent, eoff, two, s0, zero, wave,
+                     seq);
   return hipGetLastError();
 }
 
