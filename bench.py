@@ -373,17 +373,22 @@ def main():
     print(f"bench: workload built in {time.perf_counter() - t0:.1f} s: {setup.n_nodes} nodes, "
           f"{setup.n_slots()} directed bonds", file=sys.stderr, flush=True)
     dim = setup.dim
-    # Verlet skin and block size from the autotuner, run on a separate handle
-    # of the same workload so that the timed steps are the same ones
-    if args.tuning:
-        skin, block = float(args.tuning.split(",")[0]), int(args.tuning.split(",")[1])
-    else:
-        t0 = time.perf_counter()
-        with Stepper(setup, params, device=0) as tune:
-            skin, block = tune.autotune(trial_steps=128)
-        print(f"bench: autotuned skin factor {skin}, block {block} in "
-              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     with Stepper(setup, params, device=0) as g:
+        # Verlet skin and block size from the autotuner, run on a separate
+        # handle of the same workload so that the timed steps are the same
+        # ones.  The timed handle is set up first: the autotuner's trial
+        # steps then keep the GPU busy up to the warm-up, so the timed steps
+        # do not start at the clocks of a GPU that idled through a host-side
+        # set-up (the step kernel runs about 10% slower for the first ~100 ms
+        # of load after an idle second).
+        if args.tuning:
+            skin, block = float(args.tuning.split(",")[0]), int(args.tuning.split(",")[1])
+        else:
+            t0 = time.perf_counter()
+            with Stepper(setup, params, device=0) as tune:
+                skin, block = tune.autotune(trial_steps=128)
+            print(f"bench: autotuned skin factor {skin}, block {block} in "
+                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         g.set_tuning(skin, block)
         g.step(args.warmup)
         g.sync()

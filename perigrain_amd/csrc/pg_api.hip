@@ -1143,6 +1143,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
   if (nsteps == 0) return PG_OK;
   const uint32_t t_end = h->t + nsteps;
   if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
+  bool react_queued = false; // the last step's reaction is in the stream behind it
   while (h->t < t_end) {
     uint32_t tr = next_reset(h);
     if (tr == h->t + 1) {
@@ -1184,6 +1185,13 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
       }
     }
     HIPCHK(hipEventRecord(h->kev[1], h->stream));
+    // a plain run's last chunk: its recorded step's reaction goes into the
+    // stream now, so the wait below covers it; if the chunk turns out to
+    // have stopped early, the redone steps queue it again
+    react_queued = !h->hist && h->t + chunk == t_end;
+    if (react_queued)
+      RC(reactions(h, h->wterm, h->wflag, h->rhits, h->rcnt, 1, 0, 1, h->reaction,
+                   (size_t)h->I * 18));
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1204,6 +1212,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
       if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid before the batch started");
       h->skipped += (uint64_t)(h->t + chunk) - v + 1;
       h->t = v - 1;
+      react_queued = false;
       RC(build_list(h, v));
       continue;
     }
@@ -1213,7 +1222,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     HIPCHK(hipMemcpyAsync(h->reaction, h->hist + (size_t)(t_end - h->hist_t0 - 1) * h->I * 18,
                           (size_t)h->I * 18 * sizeof(double), hipMemcpyDeviceToDevice,
                           h->stream));
-  else
+  else if (!react_queued)
     RC(reactions(h, h->wterm, h->wflag, h->rhits, h->rcnt, 1, 0, 1, h->reaction, (size_t)h->I * 18));
   return PG_OK;
 }
