@@ -223,6 +223,28 @@ def test_3d_compression_long_horizon_bit_exact(oracle_threads):
     assert r[5, 2] > 0, "top wall should push"
 
 
+def test_3d_wall_reaction_per_call_across_list_rebuilds(oracle_threads):
+    # pg_step queues its last step's wall reaction before it learns whether
+    # the batch stopped early on a stale neighbour list; then the redone
+    # steps must queue it again.  Fast grains and a small Verlet skin force
+    # rebuilds inside short calls; after every call the reaction (and the
+    # state) must equal the oracle's at that step.
+    s, p = setups.compression_3d(seed=5, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-20.0)
+    rng = np.random.default_rng(3)
+    for gi in range(s.n_grains):
+        s.vel[s.grain_nodes(gi)] = rng.normal(size=3) * 20.0
+    g, o = _both(s, p, oracle_threads)
+    g.set_tuning(0.25, 256)
+    redone_calls = 0
+    for n in (1, 1, 3, 7, 17, 40, 29, 64, 65, 33, 50):
+        before = g.stats()["skipped_steps"]
+        _advance(g, o, n)
+        redone_calls += g.stats()["skipped_steps"] > before
+        _assert_exact(g, o, f"t={g.t}")
+    assert g.stats()["rebuilds"] > 3, "the grains should force list rebuilds"
+    assert redone_calls > 0, "some call should stop on a stale list and redo its steps"
+
+
 def _falling_disk(seed, reset_step):
     # disk 0 starts 4R above disk 1 and falls at 50 m/s; at reset_step
     # reset_partzero_y moves it (pos shifted in place, bonds unchanged) to
