@@ -957,17 +957,19 @@ __global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
 }
 
 // k_reaction: one block per recorded step.  The compacted hits are taken in
-// pieces of REACT_PIECE, one per thread.  The block stages a piece: the four
-// terms of every side each hit touched -- normal, damping, friction along the
-// side's two tangents -- go into one list per side, in key order (ballots
-// give the positions).  Each (instance,
-// side, component) is a chain of subtractions in key order: terms 0 and 1 of
-// every entry along the side's direction, term 2 or 3 across it.  A node that
-// did not touch a side adds nothing to it (in the reference that side is not
-// updated; here it is not visited).  Running sums stay in LDS between pieces
-// (the first REACT_LDS_SUMS of them; the rest in the output).  Each thread
-// loads its hit of the next piece (key, sides, terms) before the current
-// piece is summed, so those loads overlap the sums.
+// pieces of T, one per thread, T being the block size: 512 for one step's
+// reaction, whose wall may hold a thousand hits, 256 for a ring of recorded
+// steps, one block per step with a few hits each.  The block stages a piece:
+// the four terms of every side each hit touched -- normal, damping, friction
+// along the side's two tangents -- go into one list per side, in key order
+// (ballots give the positions).  Each (instance, side, component) is a chain
+// of subtractions in key order: terms 0 and 1 of every entry along the
+// side's direction, term 2 or 3 across it.  A node that did not touch a side
+// adds nothing to it (in the reference that side is not updated; here it is
+// not visited).  Running sums stay in LDS between pieces (the first
+// REACT_LDS_SUMS of them; the rest in the output).  Each thread loads its hit
+// of the next piece (key, sides, terms) before the current piece is summed,
+// so those loads overlap the sums.
 //
 // A chain of at most REACT_SEQ_MAX terms is summed by one thread in order.
 // A longer one -- one wall pressed by many nodes -- would be a long chain of
@@ -977,8 +979,6 @@ __global__ __launch_bounds__(256) void k_wall_compact(ReactionArgs R) {
 // then every chunk repeated the sequential subtractions from the sequential
 // start value, and the last chunk's end is the sequential result, bit for
 // bit.
-constexpr uint32_t REACT_PIECE = 512;      // hits staged in LDS at a time, one per thread
-constexpr uint32_t REACT_THREADS = REACT_PIECE;
 constexpr uint32_t REACT_SEQ_MAX = 64;     // longest chain summed by one thread
 constexpr uint32_t REACT_LONG_MAX = 192;   // longer chains of a piece handed to wavefronts
 constexpr uint32_t REACT_LDS_SUMS = 18 * 64; // running sums held in LDS
@@ -1132,8 +1132,8 @@ __device__ __forceinline__ double react_chain_wave(const ChainTerms &ct, uint32_
   return read_lane_d(end, C - 1);
 }
 
-template <int D>
-__global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
+template <int D, uint32_t T>
+__global__ __launch_bounds__(T) void k_reaction(ReactionArgs R) {
   const uint32_t slot = (R.first + blockIdx.y) % R.slots;
   const double *wterm = R.wterm + slot * R.wstride;
   const uint32_t *hits = R.hits + blockIdx.y * R.hstride;
@@ -1141,13 +1141,13 @@ __global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
   double *reaction = R.reaction + blockIdx.y * R.rstride;
   constexpr int NS = (D == 2) ? 4 : 6;
   constexpr uint32_t PER_INST = NS * D;
-  constexpr uint32_t NW = REACT_THREADS / 64;
-  constexpr uint32_t ZERO = REACT_PIECE; // index of each side list's all-zero entry
-  __shared__ uint32_t s_off[REACT_THREADS + 1];
+  constexpr uint32_t NW = T / 64;
+  constexpr uint32_t ZERO = T; // index of each side list's all-zero entry
+  __shared__ uint32_t s_off[T + 1];
   __shared__ uint32_t wtot[NW];
   __shared__ uint32_t s_wcnt[NW][NS]; // hits of a piece per wavefront and side
-  __shared__ uint32_t s_key[NS][REACT_PIECE];
-  __shared__ double2 s_side[NS][REACT_PIECE + 1][2]; // terms (0, 1), (2, 3) of one side
+  __shared__ uint32_t s_key[NS][T];
+  __shared__ double2 s_side[NS][T + 1][2]; // terms (0, 1), (2, 3) of one side
   __shared__ double s_acc[REACT_LDS_SUMS];
   __shared__ uint32_t s_long[REACT_LONG_MAX][3]; // assignment, first entry, entry end
   __shared__ uint32_t s_nlong;
@@ -1169,14 +1169,14 @@ __global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
     return ct;
   };
   // a thread's short-chain sums are the assignments a = tid (mod
-  // REACT_THREADS): it alone clears, updates and writes them; a long chain's
+  // T): it alone clears, updates and writes them; a long chain's
   // sum is read and written by a wavefront between two barriers
-  for (uint32_t a = tid; a < n_assign; a += REACT_THREADS) {
+  for (uint32_t a = tid; a < n_assign; a += T) {
     if (a < REACT_LDS_SUMS) s_acc[a] = 0.0;
     else *out_at(a) = 0.0;
   }
   if (tid < 2 * NS) s_side[tid >> 1][ZERO][tid & 1] = make_double2(0.0, 0.0);
-  for (uint32_t g0 = 0; g0 < R.nchunks; g0 += REACT_THREADS) {
+  for (uint32_t g0 = 0; g0 < R.nchunks; g0 += T) {
     // offsets of this group's chunks in the key-ordered list of hits
     const uint32_t c = (g0 + tid < R.nchunks) ? ccnt[g0 + tid] : 0u;
     uint32_t v = c;
@@ -1194,11 +1194,11 @@ __global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
       total += wtot[q];
     }
     s_off[tid] = excl;
-    if (tid == 0) s_off[REACT_THREADS] = total;
+    if (tid == 0) s_off[T] = total;
     __syncthreads();
     ReactHit<NS> h;
-    react_fetch<NS>(hits, wterm, s_off, REACT_THREADS, g0, total, tid, h);
-    for (uint32_t p0 = 0; p0 < total; p0 += REACT_PIECE) {
+    react_fetch<NS>(hits, wterm, s_off, T, g0, total, tid, h);
+    for (uint32_t p0 = 0; p0 < total; p0 += T) {
       // positions in the per-side lists: per wavefront, the hits of lower
       // lanes (ballot and bit count); across wavefronts, their totals
       uint32_t wpos[NS];
@@ -1222,8 +1222,8 @@ __global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
         }
       __syncthreads();
       // the next piece's loads overlap the sums below
-      react_fetch<NS>(hits, wterm, s_off, REACT_THREADS, g0, total, p0 + REACT_PIECE + tid, h);
-      for (uint32_t a = tid; a < n_assign; a += REACT_THREADS) {
+      react_fetch<NS>(hits, wterm, s_off, T, g0, total, p0 + T + tid, h);
+      for (uint32_t a = tid; a < n_assign; a += T) {
         const uint32_t inst = a / PER_INST, sd = (a / D) % NS;
         uint32_t nside = 0;
         for (uint32_t q = 0; q < NW; ++q) nside += s_wcnt[q][sd];
@@ -1279,7 +1279,7 @@ __global__ __launch_bounds__(REACT_THREADS) void k_reaction(ReactionArgs R) {
       __syncthreads();
     }
   }
-  for (uint32_t a = tid; a < n_assign && a < REACT_LDS_SUMS; a += REACT_THREADS)
+  for (uint32_t a = tid; a < n_assign && a < REACT_LDS_SUMS; a += T)
     *out_at(a) = s_acc[a];
 }
 
@@ -1682,8 +1682,14 @@ hipError_t launch_reactions(int dim, const ReactionArgs &R, uint32_t nsteps, hip
   if (nsteps == 0) return hipSuccess;
   if (R.nchunks > 0)
     hipLaunchKernelGGL(k_wall_compact, dim3(R.nchunks, nsteps), dim3(256), 0, s, R);
-  if (dim == 2) hipLaunchKernelGGL(k_reaction<2>, dim3(1, nsteps), dim3(REACT_THREADS), 0, s, R);
-  else hipLaunchKernelGGL(k_reaction<3>, dim3(1, nsteps), dim3(REACT_THREADS), 0, s, R);
+  // one step: pieces of 512; a ring of steps: one block of 256 per step
+  if (nsteps == 1) {
+    if (dim == 2) hipLaunchKernelGGL((k_reaction<2, 512>), dim3(1, 1), dim3(512), 0, s, R);
+    else hipLaunchKernelGGL((k_reaction<3, 512>), dim3(1, 1), dim3(512), 0, s, R);
+  } else {
+    if (dim == 2) hipLaunchKernelGGL((k_reaction<2, 256>), dim3(1, nsteps), dim3(256), 0, s, R);
+    else hipLaunchKernelGGL((k_reaction<3, 256>), dim3(1, nsteps), dim3(256), 0, s, R);
+  }
   return hipGetLastError();
 }
 
