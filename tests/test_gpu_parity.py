@@ -245,6 +245,36 @@ def test_3d_wall_reaction_per_call_across_list_rebuilds(oracle_threads):
     assert redone_calls > 0, "some call should stop on a stale list and redo its steps"
 
 
+def test_3d_batched_wall_reaction_long_chains_equal_single_runs():
+    # three 3D compressions in one handle: each instance's top-wall list in
+    # the reaction kernel's piece is a range of the shared side list, long
+    # enough to be summed by a wavefront; every instance's reaction (and
+    # state) must equal its own single-instance run bit for bit
+    sp = [setups.compression_3d(seed=sd, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-20.0)
+          for sd in (5, 6, 7)]
+    ss = [s for s, _ in sp]
+    ps = [p for _, p in sp]
+    steps = 400
+    with Stepper(ss, ps, device=0) as b:
+        b.step(steps)
+        long_lists = 0
+        for k, (s, p) in enumerate(sp):
+            x = b.read("CurrPos", inst=k)
+            wpos, _ = b.wall(inst=k)
+            hits = int((np.abs(x[:, 2] - wpos[5]) <= s.contact_radius).sum())
+            long_lists += hits > 32  # direction chain of more than 64 terms
+            with Stepper(s, p, device=0) as one:
+                one.step(steps)
+                for f in FIELDS:
+                    assert np.array_equal(b.read(f, inst=k), one.read(f)), f"instance {k} {f}"
+                w1, r1 = one.wall()
+                wb, rb = b.wall(inst=k)
+                assert np.array_equal(wb, w1), f"instance {k} wall positions"
+                assert np.array_equal(rb, r1), f"instance {k} wall reaction {rb[5]} vs {r1[5]}"
+                assert rb[5, 2] != 0, f"instance {k}: the top wall should push"
+        assert long_lists >= 2, "the top walls should hold long lists"
+
+
 def _falling_disk(seed, reset_step):
     # disk 0 starts 4R above disk 1 and falls at 50 m/s; at reset_step
     # reset_partzero_y moves it (pos shifted in place, bonds unchanged) to
