@@ -156,6 +156,33 @@ __device__ __forceinline__ void bond_term(uint32_t cw, const Nbr<D> &nb, const d
   }
 }
 
+// bond_term's arithmetic without the sum: p = w*e, the term bond_term adds
+// to F (F[q] + p[q] is F[q] + w*e[q] with the same two roundings).
+template <int D, bool CHECK>
+__device__ __forceinline__ bool bond_prod(uint32_t cw, const Nbr<D> &nb, const double *Xo,
+                                          const double *ui, double cnot, double snot, bool brk,
+                                          double *p) {
+  const bool live = !(cw & BCOL_BROKEN);
+  double xi[D], eta[D], e[D], s;
+#pragma unroll
+  for (int q = 0; q < D; ++q) xi[q] = nb.X[q] - Xo[q];
+#pragma unroll
+  for (int q = 0; q < D; ++q) eta[q] = (nb.u[q] - ui[q]) + xi[q];
+  bond_geom<D, CHECK>(xi, eta, live, s, e);
+  const double w = live ? (cnot * s) * nb.V : 0.0;
+#pragma unroll
+  for (int q = 0; q < D; ++q) p[q] = w * e[q];
+  return live && brk && s > snot;
+}
+
+// A bond whose stretch exceeded s0 gets the broken flag in its column word
+// (and, in a recorded run, its break step); bond_term does the same inline.
+__device__ __forceinline__ void bond_break(uint32_t cw, uint32_t *slot, const StepArgs &A,
+                                           uint32_t t) {
+  *slot = cw | BCOL_BROKEN;
+  if (A.break_step) A.break_step[slot - A.bcol] = t;
+}
+
 // Contact gates of a list entry (the gates of timeloop.h:698-747 and the
 // self-contact rules of contact.h:686-733): partner j of grain gj, reference
 // position Xj, bond slot bs (-1: no bond, or one known to be broken).
@@ -387,6 +414,96 @@ __device__ __forceinline__ void bond_loop(const StepArgs &A, uint32_t t, uint32_
   }
 }
 
+// Lane j of a node's LPN lanes (LPN = 2 or 4, neighbouring lanes of one
+// quad) as a DPP source: every lane of the node's group reads lane j's x.
+template <int LPN>
+__device__ __forceinline__ double node_lane_d(double x, int j) {
+  // quad_perm patterns: LPN 4 reads lane j of the quad; LPN 2 lane j of
+  // each pair
+  const long long b = __double_as_longlong(x);
+  int lo, hi;
+#define PG_QP(c)                                                                      \
+  lo = __builtin_amdgcn_update_dpp(0, (int)b, (c), 0xF, 0xF, false);                 \
+  hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), (c), 0xF, 0xF, false)
+  if (LPN == 4) {
+    switch (j) {
+    case 0: PG_QP(0x00); break;
+    case 1: PG_QP(0x55); break;
+    case 2: PG_QP(0xAA); break;
+    default: PG_QP(0xFF); break;
+    }
+  } else {
+    if (j == 0) { PG_QP(0xA0); } // lanes 0,0,2,2
+    else { PG_QP(0xF5); }        // lanes 1,1,3,3
+  }
+#undef PG_QP
+  return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
+// The bond loop of node i shared by its LPN lanes (small workloads, where one
+// lane per node would leave most of the GPU idle): lane `sub` evaluates slots
+// j = sub, sub + LPN, ... of each four-slot group, and every lane of the node
+// then adds the group's four terms to F in slot order, each read from the lane
+// that evaluated it -- the same additions in the same order as bond_loop, so
+// all LPN lanes hold bond_loop's sum.  Column words go out two groups ahead,
+// partner records one group ahead.
+template <int D, bool CHECK, int LPN, class RS>
+__device__ __forceinline__ void bond_loop_lanes(const StepArgs &A, uint32_t t, uint32_t i,
+                                                uint32_t sub, const RS &rs, const double *Xo,
+                                                const double *ui, double cnot, double snot,
+                                                double gvol, bool brk, double *F, bool &broke) {
+  constexpr int PER = BGROUP / LPN; // slots per lane per group
+  const uint32_t sl = i >> 6, lane = i & 63;
+  const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
+  uint32_t *bc = A.bcol + base + BGROUP * lane;
+  // the column array has BCOL_TAIL spare slots after the last slice, so
+  // reading two groups past a row's end stays in bounds
+  auto colw = [&](uint32_t q, int p) -> uint32_t {
+    const uint32_t w = bc[(SLICE * BGROUP) * q + sub + LPN * p];
+    return (BGROUP * q < width) ? w : BCOL_PAD;
+  };
+  uint32_t c0[PER], c1[PER], c2[PER];
+  Nbr<D> r0[PER], r1[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    c0[p] = colw(0, p);
+    c1[p] = colw(1, p);
+  }
+#pragma unroll
+  for (int p = 0; p < PER; ++p) rs.load(c0[p], i, gvol, r0[p]);
+  for (uint32_t q = 0; BGROUP * q < width; ++q) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) c2[p] = colw(q + 2, p);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) rs.load(c1[p], i, gvol, r1[p]);
+    double pr[PER][D];
+    bool bk = false, bks[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      bks[p] = bond_prod<D, CHECK>(c0[p], r0[p], Xo, ui, cnot, snot, brk, pr[p]);
+      bk = bk || bks[p];
+    }
+    // slot j of the group: lane j % LPN, its p = j / LPN
+#pragma unroll
+    for (int j = 0; j < BGROUP; ++j)
+#pragma unroll
+      for (int k = 0; k < D; ++k) F[k] = F[k] + node_lane_d<LPN>(pr[j / LPN][k], j % LPN);
+    if (bk) { // rare
+      uint32_t *sq = bc + (SLICE * BGROUP) * q;
+#pragma unroll
+      for (int p = 0; p < PER; ++p)
+        if (bks[p]) bond_break(c0[p], sq + sub + LPN * p, A, t);
+      broke = true;
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      c0[p] = c1[p];
+      c1[p] = c2[p];
+      r0[p] = r1[p];
+    }
+  }
+}
+
 // ------------------------------------------------------------ step kernel
 
 // One full time step t for every node (timeloop.h:608-878):
@@ -460,9 +577,10 @@ __device__ __forceinline__ void step_housekeeping(const StepArgs &A, uint32_t t,
 
 // The step of node i (all lanes of a wavefront call it together: the grain
 // boxes are combined across the wavefront).
-template <int D, bool RECORD, bool UVOL, class RS>
+template <int D, bool RECORD, bool UVOL, int LPN = 1, class RS>
 __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, const uint32_t i,
-                                          const RS &rs, BlockBox *bb = nullptr) {
+                                          const RS &rs, BlockBox *bb = nullptr,
+                                          uint32_t sub = 0) {
   const uint32_t ns = A.ns;
   const bool valid = i < A.n_nodes;
   uint32_t g = 0xFFFFFFFFu;
@@ -497,10 +615,24 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
         const double cnot = gq->cnot, snot = gq->snot;
         const double gvol = UVOL ? gq->vol : 0.0;
         const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
-        if (A.unice[t % 3] == 0 && A.xi_ok) // the same for every lane of the launch
-          bond_loop<D, false>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
-        else
-          bond_loop<D, true>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
+        if constexpr (LPN == 1) {
+          if (A.unice[t % 3] == 0 && A.xi_ok) // the same for every lane of the launch
+            bond_loop<D, false>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
+          else
+            bond_loop<D, true>(A, t, i, rs, Xo, ui, cnot, snot, gvol, brk, F, broke);
+        } else {
+          if (A.unice[t % 3] == 0 && A.xi_ok)
+            bond_loop_lanes<D, false, LPN>(A, t, i, sub, rs, Xo, ui, cnot, snot, gvol, brk, F,
+                                           broke);
+          else
+            bond_loop_lanes<D, true, LPN>(A, t, i, sub, rs, Xo, ui, cnot, snot, gvol, brk, F,
+                                          broke);
+          // a break in any of the node's lanes (the node's lanes are all
+          // active here: they share its conditions)
+          const unsigned long long m = __ballot(broke);
+          const uint32_t l0 = (threadIdx.x & 63) & ~(uint32_t)(LPN - 1);
+          broke = ((m >> l0) & ((1ull << LPN) - 1)) != 0;
+        }
         if (broke) A.brk_new[i] = 1;
       }
     }
@@ -714,20 +846,25 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
   else wave_box_atomic<D>(A.bbox[(t + 1) % 3], g, valid, xn);
 }
 
-template <int D, bool RECORD, bool UVOL>
+// LPN lanes per node (1, 2 or 4): with more than one, the node's lanes share
+// its bond loop (bond_loop_lanes) and repeat the rest of its step, which
+// writes the same values and whose atomics (min, max, or) are idempotent.
+template <int D, bool RECORD, bool UVOL, int LPN>
 __global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
                                                                    const uint32_t t_arg) {
   const uint32_t t = step_index(A, t_arg);
   if (*A.violation <= t) return; // neighbour list stale since an earlier step
   const uint32_t nb = gridDim.x;
   const uint32_t lb = xcd_block(blockIdx.x, nb);
-  const uint32_t i = lb * blockDim.x + threadIdx.x;
+  const uint32_t gt = lb * blockDim.x + threadIdx.x;
+  const uint32_t i = gt / LPN;
   __shared__ BlockBox bb;
-  block_box_init<D>(bb, A, lb * blockDim.x);
+  block_box_init<D>(bb, A, lb * blockDim.x / LPN);
   __syncthreads();
-  step_housekeeping<D>(A, t, i, nb * blockDim.x);
-  step_node<D, RECORD, UVOL>(A, t, i, GlobalRecs<D, UVOL>(A.rec[t & 1], A.ns, A.V, A.ghost),
-                             &bb);
+  step_housekeeping<D>(A, t, gt, nb * blockDim.x);
+  step_node<D, RECORD, UVOL, LPN>(A, t, i,
+                                  GlobalRecs<D, UVOL>(A.rec[t & 1], A.ns, A.V, A.ghost), &bb,
+                                  gt % LPN);
 }
 
 // The same step with each workgroup's partner window staged in LDS: block b
@@ -1614,8 +1751,24 @@ static void launch_step_t(bool uvol, const StepArgs &A, uint32_t t, uint32_t nb,
     else hipLaunchKernelGGL((k_step_lds<D, RECORD, false>), g, b, A.lds_bytes, s, A, t);
     return;
   }
-  if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true>), dim3(nb), dim3(block), 0, s, A, t);
-  else hipLaunchKernelGGL((k_step<D, RECORD, false>), dim3(nb), dim3(block), 0, s, A, t);
+  // lanes per node for small workloads, where one lane per node leaves most
+  // SIMDs without a wavefront: four up to 16k nodes (at most one wavefront
+  // per SIMD), two up to 64k.  Measured against one lane per node: 1.3x on
+  // 1.4k and 15k-node 2D runs and 1.7x on a 3.9k-node 3D run; at 122k nodes
+  // (two wavefronts per SIMD already) two lanes per node were 9% slower.
+  const uint32_t lpn = A.n_nodes <= 16384u ? 4u : (A.n_nodes <= 65536u ? 2u : 1u);
+  const uint32_t nbl = (uint32_t)nblk((uint64_t)A.n_nodes * lpn, block);
+  const dim3 g(nbl > 0 ? nbl : 1), b(block);
+  if (lpn == 4) {
+    if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true, 4>), g, b, 0, s, A, t);
+    else hipLaunchKernelGGL((k_step<D, RECORD, false, 4>), g, b, 0, s, A, t);
+  } else if (lpn == 2) {
+    if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true, 2>), g, b, 0, s, A, t);
+    else hipLaunchKernelGGL((k_step<D, RECORD, false, 2>), g, b, 0, s, A, t);
+  } else {
+    if (uvol) hipLaunchKernelGGL((k_step<D, RECORD, true, 1>), dim3(nb), b, 0, s, A, t);
+    else hipLaunchKernelGGL((k_step<D, RECORD, false, 1>), dim3(nb), b, 0, s, A, t);
+  }
 }
 
 template <int D>

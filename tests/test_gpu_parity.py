@@ -245,6 +245,26 @@ def test_3d_wall_reaction_per_call_across_list_rebuilds(oracle_threads):
     assert redone_calls > 0, "some call should stop on a stale list and redo its steps"
 
 
+@pytest.mark.parametrize("n_grains,box", [(100, (20e-3, 40e-3))])
+def test_two_lanes_per_node_mid_size_2d(oracle_threads, n_grains, box):
+    # 30k nodes: the step kernel gives each node two lanes (16k < N <= 64k),
+    # which share its bond loop; grains kicked to 50 m/s so that contacts,
+    # walls and list rebuilds all happen within the horizon
+    s, p = setups.bulk_settle_2d(seed=2, n_grains=n_grains, box=box)
+    assert 16384 < s.n_nodes <= 65536
+    rng = np.random.default_rng(4)
+    for gi in range(s.n_grains):
+        a = rng.uniform(0, 2 * np.pi)
+        s.vel[s.grain_nodes(gi)] = 50.0 * np.array([np.cos(a), np.sin(a)])
+    g, o = _both(s, p, oracle_threads)
+    done = 0
+    for t in (1, 100, 300):
+        _advance(g, o, t - done)
+        done = t
+        _assert_close(g, o, 1e-10, f"t={t}")
+    assert len(o.contacts()) > 0, "grains should touch"
+
+
 def test_3d_batched_wall_reaction_long_chains_equal_single_runs():
     # three 3D compressions in one handle: each instance's top-wall list in
     # the reaction kernel's piece is a range of the shared side list, long
