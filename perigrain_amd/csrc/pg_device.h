@@ -377,6 +377,51 @@ __device__ __forceinline__ void pair_force(const T *d, T r, double R, T K, doubl
   }
 }
 
+// pair_force's three terms without the sums, for callers that add them
+// later in pair_force's order: F += fc, then F += fd, then F += ff.  A term
+// pair_force does not add is +0 here; adding +0 leaves those sums unchanged
+// (they start at +0, so they are never -0).  The operations are pair_force's
+// on doubles.
+template <int D>
+__device__ __forceinline__ void pair_terms(const double *d, double r, double R, double K,
+                                           double vj, const double *w, double rho, double mu,
+                                           const InstDev &ip, double *fc, double *fd, double *ff) {
+  double e[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) e[k] = d[k] / r;
+  const double g = -K * (R - r) * vj;
+#pragma unroll
+  for (int k = 0; k < D; ++k) fc[k] = g * e[k];
+  const double fcn = sqrt(red<D>(fc, fc));
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    fd[k] = 0.0;
+    ff[k] = 0.0;
+  }
+  if (ip.flags & IF_DAMPING) {
+    const double p = red<D>(w, e);
+    const double cd = 2 * ip.zeta * sqrt(K * rho) * sqrt(vj);
+#pragma unroll
+    for (int k = 0; k < D; ++k) fd[k] = (-cd) * (p * e[k]);
+  }
+  if (ip.flags & IF_FRICTION) {
+    const double p = red<D>(w, e);
+    double tt[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) tt[k] = w[k] - p * e[k];
+    const double tau = sqrt(red<D>(tt, tt));
+    if (tau > 0) {
+      double ut[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) ut[k] = tt[k] / tau;
+      double s = -mu * fcn;
+      if (tau < mu * fcn * ip.dt / rho) s = -tau / rho / ip.dt;
+#pragma unroll
+      for (int k = 0; k < D; ++k) ff[k] = s * ut[k];
+    }
+  }
+}
+
 } // namespace pg
 
 #endif

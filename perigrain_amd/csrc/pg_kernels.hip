@@ -677,6 +677,54 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
         // the next fast-list entry is loaded while the current one is
         // evaluated, and a partner's data is loaded before the gates decide
         // whether it is needed (one round trip to memory per entry)
+        if constexpr (LPN > 1) {
+          // the node's lanes take consecutive entries; each lane's three
+          // terms are then added in entry order on every lane.  A bond
+          // broken in this step's bond loop by another of the node's lanes
+          // is visible here: its store is complete before these loads.
+          if (broke) __builtin_amdgcn_s_waitcnt(0);
+          for (uint32_t cb = c0; cb < c1; cb += LPN) {
+            const uint32_t c = cb + sub;
+            double fc[D], fd[D], ff[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) fc[k] = fd[k] = ff[k] = 0.0;
+            if (c < c1) {
+              uint32_t j, gj;
+              int32_t bs;
+              if (full) {
+                j = A.cl_j[c];
+                gj = A.node_grain[j];
+                bs = A.cl_slot[c];
+              } else {
+                const uint2 en = A.cl_fast[c];
+                j = en.x;
+                gj = en.y;
+                bs = -1;
+              }
+              double Xj[D], uj[D], vj[D];
+              pos_X<D>(A, ip, gj, j, Xj);
+              rec_u<D>(U, A.ns, j, uj);
+              load_vec<D>(vprev, ns, j, vj);
+              const double Vj = A.V[j];
+              double d[D], r, R, K;
+              if (contact_gate<D>(A, t, gj, bs, g, gp, ip, Xc, Xj, selfc, stop, boxg, R, K) &&
+                  pair_within<D>(Xj, uj, xc, R, d, r)) {
+                double w[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) w[k] = vi[k] - vj[k];
+                pair_terms<D>(d, r, R, K, Vj, w, gp.rho, ip.mu, ip, fc, fd, ff);
+              }
+            }
+#pragma unroll
+            for (int jl = 0; jl < LPN; ++jl)
+#pragma unroll
+              for (int k = 0; k < D; ++k) {
+                F[k] = F[k] + node_lane_d<LPN>(fc[k], jl);
+                F[k] = F[k] + node_lane_d<LPN>(fd[k], jl);
+                F[k] = F[k] + node_lane_d<LPN>(ff[k], jl);
+              }
+          }
+        } else {
         uint2 nxt = (!full && c0 < c1) ? A.cl_fast[c0] : make_uint2(0u, 0u);
         for (uint32_t c = c0; c < c1; ++c) {
           uint32_t j, gj;
@@ -704,6 +752,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 #pragma unroll
           for (int k = 0; k < D; ++k) w[k] = vi[k] - vj[k];
           pair_force<D, double>(d, r, R, K, Vj, w, gp.rho, ip.mu, ip, F);
+        }
         }
       }
 
