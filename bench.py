@@ -376,31 +376,37 @@ def main():
     with Stepper(setup, params, device=0) as g:
         # Verlet skin and block size from the autotuner, run on a separate
         # handle of the same workload so that the timed steps are the same
-        # ones.  The timed handle is set up first: the autotuner's trial
-        # steps then keep the GPU busy up to the warm-up, so the timed steps
-        # do not start at the clocks of a GPU that idled through a host-side
-        # set-up (the step kernel runs about 10% slower for the first ~100 ms
-        # of load after an idle second).
+        # ones.  The timed handle is set up first and the autotuner's handle
+        # is closed only after the timed steps: the trial steps then keep
+        # the GPU busy up to the warm-up, so the timed steps do not start at
+        # the clocks of a GPU that idled through a host-side set-up or
+        # teardown (the step kernel runs 5-10% slower for the first tens of
+        # milliseconds of load after the GPU idled for a few milliseconds).
+        tune = None
         if args.tuning:
             skin, block = float(args.tuning.split(",")[0]), int(args.tuning.split(",")[1])
         else:
             t0 = time.perf_counter()
-            with Stepper(setup, params, device=0) as tune:
-                skin, block = tune.autotune(trial_steps=128)
+            tune = Stepper(setup, params, device=0)
+            skin, block = tune.autotune(trial_steps=128)
             print(f"bench: autotuned skin factor {skin}, block {block} in "
                   f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-        g.set_tuning(skin, block)
-        g.step(args.warmup)
-        g.sync()
-        st0 = g.stats()
-        k0 = g.kernel_time()
-        g.event(0)
-        g.step(args.steps)
-        g.event(1)
-        ms = g.elapsed_ms(0, 1)
-        g.sync()
-        st1 = g.stats()
-        k1 = g.kernel_time()
+        try:
+            g.set_tuning(skin, block)
+            g.step(args.warmup)
+            g.sync()
+            st0 = g.stats()
+            k0 = g.kernel_time()
+            g.event(0)
+            g.step(args.steps)
+            g.event(1)
+            ms = g.elapsed_ms(0, 1)
+            g.sync()
+            st1 = g.stats()
+            k1 = g.kernel_time()
+        finally:
+            if tune is not None:
+                tune.close()
     n_nodes, n_slots = setup.n_nodes, setup.n_slots()
     ms_per_step = ms / args.steps
     value = n_nodes * args.steps / (ms / 1e3)
