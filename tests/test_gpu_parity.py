@@ -295,6 +295,48 @@ def test_3d_batched_wall_reaction_long_chains_equal_single_runs():
         assert long_lists >= 2, "the top walls should hold long lists"
 
 
+def test_3d_batched_instances_against_the_oracle(oracle_threads):
+    """Three 3D compressions of different seeds, wall speeds and sizes in one
+    handle, one with a sphere expanding so that its bonds break: every
+    instance against the oracle's run of it, bit for bit (fields, bond
+    status, contact pairs, wall positions and reaction) at three steps."""
+    sp = [setups.compression_3d(seed=11, n_side=2, R=0.5e-3, h_div=5, cube=3e-3, wall_speed=-20.0),
+          setups.compression_3d(seed=12, n_side=2, R=0.6e-3, h_div=5, cube=3.5e-3, wall_speed=-5.0),
+          setups.compression_3d(seed=13, n_side=3, R=0.4e-3, h_div=4, cube=4e-3, wall_speed=-30.0)]
+    s1 = sp[1][0]
+    nodes = s1.grain_nodes(2)
+    s1.vel[nodes] += 6e4 * (s1.pos[nodes] - s1.pos[nodes].mean(axis=0))
+    ss = [s for s, _ in sp]
+    ps = [p for _, p in sp]
+    print(f"instances: {[s.n_nodes for s in ss]} nodes", flush=True)
+    os_ = [oracle.Oracle(s, p, threads=oracle_threads) for s, p in sp]
+    try:
+        with Stepper(ss, ps, device=0) as b:
+            done = 0
+            for t in (1, 60, 150):
+                b.step(t - done)
+                for k, o in enumerate(os_):
+                    o.step(t - done, record_contacts=True)
+                    for f in FIELDS:
+                        a, c = b.read(f, inst=k), o.read(f)
+                        assert np.array_equal(a, c), \
+                            f"instance {k} t={t} {f}: {int((a != c).sum())} entries differ"
+                    assert np.array_equal(b.bond_intact(inst=k), o.bond_intact()), \
+                        f"instance {k} t={t} bond status"
+                    assert np.array_equal(b.contacts(inst=k), o.contacts()), \
+                        f"instance {k} t={t} contact pairs"
+                    wb, rb = b.wall(inst=k)
+                    wo, ro = o.wall()
+                    assert np.array_equal(wb, wo), f"instance {k} t={t} wall positions"
+                    assert np.array_equal(rb, ro), f"instance {k} t={t} wall reaction"
+                done = t
+            assert int((os_[1].bond_intact() == 0).sum()) > 100, "instance 1 should break bonds"
+            assert any(np.any(o.wall()[1] != 0) for o in os_), "a wall should push"
+    finally:
+        for o in os_:
+            o.close()
+
+
 def _falling_disk(seed, reset_step):
     # disk 0 starts 4R above disk 1 and falls at 50 m/s; at reset_step
     # reset_partzero_y moves it (pos shifted in place, bonds unchanged) to
