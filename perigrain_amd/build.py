@@ -95,6 +95,23 @@ def build_io():
     return outs
 
 
+def build_io_sanitized(out_dir):
+    """libpgio built with the address and undefined-behaviour sanitizers into
+    out_dir (host C++ only, for the CPU test that runs the file-side tests
+    against it); returns the library's path."""
+    inc, lib = find_hdf5()
+    hdir = os.path.join(BIN, "hdf5")
+    if not os.path.exists(os.path.join(hdir, os.path.basename(lib))):
+        build_io()
+    out = os.path.join(out_dir, "libpgio.so")
+    san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+           "-fno-omit-frame-pointer", "-g", "-O1"]
+    _run([CXX] + san + ["-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE, "-I", inc, "-shared",
+                        os.path.join(HOST, "pgio.cpp"), "-o", out, "-L", hdir,
+                        "-l:" + os.path.basename(lib), "-Wl,-rpath," + hdir])
+    return out
+
+
 def build_all():
     return [build_libpg()] + build_io()
 

@@ -15,7 +15,9 @@ from . import _ffi
 from .setups import Params, Setup
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPGIO_PATH = os.path.join(_HERE, "libpgio.so")
+# PERIGRAIN_LIBPGIO selects another build of the library (e.g. one built with
+# the address and undefined-behaviour sanitizers)
+LIBPGIO_PATH = os.environ.get("PERIGRAIN_LIBPGIO") or os.path.join(_HERE, "libpgio.so")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 F64, I64, U32, I32 = 0, 1, 2, 3
