@@ -169,6 +169,9 @@ __device__ __forceinline__ gbytes uniform_global(const void *p) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
   return reinterpret_cast<gbytes>(((uint64_t)hi << 32) | lo);
 }
+// Word 3 of a raw (untyped) buffer descriptor on gfx9-family GPUs: data
+// format 32 bits, no index stride, no thread-id addressing.
+constexpr int kBufWord3 = 0x00020000;
 __device__ __forceinline__ dvec2 load_dvec2(gbytes base, uint32_t off) {
   return *reinterpret_cast<const dvec2 __attribute__((address_space(1))) *>(base + off);
 }

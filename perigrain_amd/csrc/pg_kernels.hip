@@ -384,10 +384,23 @@ __device__ __forceinline__ void bond_loop(const StepArgs &A, uint32_t t, uint32_
   const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
   // column words of this lane: group q (slots 4q..4q+3) at bc + 256*q
   uint32_t *bc = A.bcol + base + BGROUP * lane;
+  // They are read by buffer loads over the slice (a descriptor with the
+  // slice's first slot as base, so offsets stay small; its range check is
+  // left open, bounds come from BCOL_TAIL as for the other loads): on the
+  // 3D compression workload 1% faster than the same loads through bc.
+  // (the wavefront's lanes share the slice, so the base is uniform and the
+  // descriptor stays in scalar registers)
+  const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+      static_cast<void *>(A.bcol + __builtin_amdgcn_readfirstlane(base)), (short)0, 0x7FFFFFFF,
+      kBufWord3);
   // the column array has BCOL_TAIL spare slots after the last slice, so
   // reading two groups past a row's end stays in bounds
   auto colg = [&](uint32_t q) -> uint4 {
-    const uint4 w = *reinterpret_cast<const uint4 *>(bc + (SLICE * BGROUP) * q);
+    typedef unsigned int uv4 __attribute__((ext_vector_type(4)));
+    const uv4 v = __builtin_bit_cast(
+        uv4, __builtin_amdgcn_raw_buffer_load_b128(
+                 brs, (BGROUP * lane + (SLICE * BGROUP) * q) * (uint32_t)sizeof(uint32_t), 0, 0));
+    const uint4 w = make_uint4(v.x, v.y, v.z, v.w);
     return (BGROUP * q < width) ? w : make_uint4(BCOL_PAD, BCOL_PAD, BCOL_PAD, BCOL_PAD);
   };
   Nbr<D> ra0, ra1, rb0, rb1;
