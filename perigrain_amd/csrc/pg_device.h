@@ -158,23 +158,10 @@ __device__ __forceinline__ uint32_t step_index(const StepArgs &A, uint32_t t_arg
   return __builtin_amdgcn_readfirstlane(A.tbase ? *A.tbase + t_arg : t_arg);
 }
 
-// Global-memory byte pointer that is the same in every lane, kept in scalar
-// registers: loads from it plus a 32-bit lane offset use the scalar-base
-// addressing of global loads (no 64-bit address arithmetic per lane).
-typedef const char __attribute__((address_space(1))) *gbytes;
 typedef double __attribute__((ext_vector_type(2))) dvec2;
-__device__ __forceinline__ gbytes uniform_global(const void *p) {
-  const uint64_t v = reinterpret_cast<uint64_t>(p);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return reinterpret_cast<gbytes>(((uint64_t)hi << 32) | lo);
-}
 // Word 3 of a raw (untyped) buffer descriptor on gfx9-family GPUs: data
 // format 32 bits, no index stride, no thread-id addressing.
 constexpr int kBufWord3 = 0x00020000;
-__device__ __forceinline__ dvec2 load_dvec2(gbytes base, uint32_t off) {
-  return *reinterpret_cast<const dvec2 __attribute__((address_space(1))) *>(base + off);
-}
 
 // Record accessors (records are [D][ns][2], rec_index).
 template <int D>

@@ -37,37 +37,6 @@ template <int D> struct Nbr {
   double u[D];
 };
 
-// Load the partner data of a column word: one 16-byte load from each record
-// part (3 in 3D, 2 in 2D), all at the byte offset 16*j from the part bases
-// (node indices stay below 2^28, so the offset fits 32 bits and the part
-// bases stay scalar).  Broken and padding slots load the ghost record (a
-// point far from every node, so their discarded term stays finite), so the
-// bond loop has no branches around its loads.  UVOL: the grain's nodes
-// share one volume, passed in gvol.
-template <int D, bool UVOL>
-__device__ __forceinline__ void load_nbr(const gbytes *P, const double *Vn, double gvol,
-                                         uint32_t cw, uint32_t ghost, Nbr<D> &nb) {
-  const uint32_t j = (cw & BCOL_BROKEN) ? ghost : cw;
-  const uint32_t off = j * 16u;
-  const dvec2 a = load_dvec2(P[0], off);
-  const dvec2 b = load_dvec2(P[1], off);
-  if constexpr (D == 3) {
-    const dvec2 c = load_dvec2(P[2], off);
-    nb.X[0] = a.x;
-    nb.X[1] = a.y;
-    nb.X[2] = b.x;
-    nb.u[0] = b.y;
-    nb.u[1] = c.x;
-    nb.u[2] = c.y;
-  } else {
-    nb.X[0] = a.x;
-    nb.X[1] = a.y;
-    nb.u[0] = b.x;
-    nb.u[1] = b.y;
-  }
-  nb.V = UVOL ? gvol : Vn[j];
-}
-
 // Length |xi|, stretch s = (|eta| - |xi|)/|xi| and direction e = eta/|eta| of
 // one bond (particle2.h:134-146), each a correctly rounded IEEE result.
 // They go through the shortened division and square-root sequences of
@@ -532,18 +501,57 @@ __device__ __forceinline__ void bond_loop_lanes(const StepArgs &A, uint32_t t, u
 // vel[(t+1)&1]; writes u^{t+1} to u[(t+1)&1] and v^t to vel[t&1].
 // Where the bond loop reads partner records from: the global record array
 // of step t, or a workgroup's window of it staged in LDS.
+// The partner data of a column word: one 16-byte load from each record part
+// (3 in 3D, 2 in 2D), all at the byte offset 16*j (node indices stay below
+// 2^28, so the offset fits 32 bits).  The loads are buffer loads, one
+// descriptor per part with a uniform base and the part's 16*ns bytes as its
+// range (an index out of range would read zeros instead of faulting); on the
+// 3D compression workload they ran 0.8% faster than global loads from scalar
+// bases.  Broken and padding slots load the ghost record (a point far from
+// every node, so their discarded term stays finite), so the bond loop has no
+// branches around its loads.  UVOL: the grain's nodes share one volume,
+// passed in gvol.
 template <int D, bool UVOL> struct GlobalRecs {
-  gbytes P[D]; // the record parts of step t
+  __amdgpu_buffer_rsrc_t R[D]; // the record parts of step t
   const double *V;
   uint32_t ghost;
   __device__ __forceinline__ GlobalRecs(const double *U, uint32_t ns, const double *V_,
                                         uint32_t ghost_)
       : V(V_), ghost(ghost_) {
+    const uint64_t bytes = (uint64_t)ns * 16u;
+    const int range = (int)(uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull);
 #pragma unroll
-    for (int q = 0; q < D; ++q) P[q] = uniform_global(U + (size_t)q * ns * 2);
+    for (int q = 0; q < D; ++q) {
+      const uint64_t a = reinterpret_cast<uint64_t>(U + (size_t)q * ns * 2);
+      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+      R[q] = __builtin_amdgcn_make_buffer_rsrc(
+          reinterpret_cast<void *>(((uint64_t)hi << 32) | lo), (short)0, range, kBufWord3);
+    }
+  }
+  __device__ __forceinline__ dvec2 part(int q, uint32_t off) const {
+    return __builtin_bit_cast(dvec2, __builtin_amdgcn_raw_buffer_load_b128(R[q], off, 0, 0));
   }
   __device__ __forceinline__ void load(uint32_t cw, uint32_t, double gvol, Nbr<D> &nb) const {
-    load_nbr<D, UVOL>(P, V, gvol, cw, ghost, nb);
+    const uint32_t j = (cw & BCOL_BROKEN) ? ghost : cw;
+    const uint32_t off = j * 16u;
+    const dvec2 a = part(0, off);
+    const dvec2 b = part(1, off);
+    if constexpr (D == 3) {
+      const dvec2 c = part(2, off);
+      nb.X[0] = a.x;
+      nb.X[1] = a.y;
+      nb.X[2] = b.x;
+      nb.u[0] = b.y;
+      nb.u[1] = c.x;
+      nb.u[2] = c.y;
+    } else {
+      nb.X[0] = a.x;
+      nb.X[1] = a.y;
+      nb.u[0] = b.x;
+      nb.u[1] = b.y;
+    }
+    nb.V = UVOL ? gvol : V[j];
   }
 };
 template <int D, bool UVOL> struct LdsRecs {
