@@ -438,10 +438,17 @@ __device__ __forceinline__ void bond_loop_lanes(const StepArgs &A, uint32_t t, u
   const uint32_t sl = i >> 6, lane = i & 63;
   const uint32_t base = A.slice_base[sl], width = A.slice_width[sl];
   uint32_t *bc = A.bcol + base + BGROUP * lane;
+  // column words by buffer loads over the slice, as in bond_loop (a
+  // wavefront's nodes lie in one slice)
+  const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+      static_cast<void *>(A.bcol + __builtin_amdgcn_readfirstlane(base)), (short)0, 0x7FFFFFFF,
+      kBufWord3);
   // the column array has BCOL_TAIL spare slots after the last slice, so
   // reading two groups past a row's end stays in bounds
   auto colw = [&](uint32_t q, int p) -> uint32_t {
-    const uint32_t w = bc[(SLICE * BGROUP) * q + sub + LPN * p];
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(
+        brs, ((SLICE * BGROUP) * q + BGROUP * lane + sub + LPN * p) * (uint32_t)sizeof(uint32_t),
+        0, 0);
     return (BGROUP * q < width) ? w : BCOL_PAD;
   };
   uint32_t c0[PER], c1[PER], c2[PER];
