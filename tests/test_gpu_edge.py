@@ -2,14 +2,16 @@
 grains of one node, grains whose nodes have no bonds, a chain grain of 65
 nodes (rows of two to six bonds across a 64-row slice boundary), a handle
 with no bonds at all, a 3D handle with a one-node grain between spheres and
-walls, and the 2D cases as instances of one batched handle.  The reference runs such grains like any other (timeloop.h:608-878
+walls, whole runs of the range-tested bond loop (a reference coordinate or a
+displacement below 2^-240), and the 2D cases as instances of one batched
+handle.  The reference runs such grains like any other (timeloop.h:608-878
 loops over each grain's nodes and NbdArr rows, however short)."""
 import numpy as np
 import pytest
 
 import oracle
 from perigrain_amd import Stepper, setups
-from test_gpu_parity import FIELDS, _advance, _assert_exact, _both
+from test_gpu_parity import FIELDS, _advance, _assert_close, _assert_exact, _both
 
 pytestmark = pytest.mark.gpu
 
@@ -66,40 +68,106 @@ def _no_bonds_2d(seed):
     return s, p
 
 
-def _one_node_3d(seed):
-    s, p = setups.compression_3d(seed=seed, n_side=2, R=0.5e-3, h_div=4, cube=3e-3,
-                                 wall_speed=-10.0)
-    h = 0.5e-3 / 4
-    g = _points_grain([[0.0, 0.0, 0.0]], h, 3, 3 * h, setups.material_3d)
-    lo = s.pos.min(axis=0)
-    # the node starts 1.5 contact radii above the bottom wall, under the
-    # lowest sphere, and falls onto the wall before rising into the spheres
-    z0 = s.wall[2] + 1.5 * s.contact_radius
-    pos = np.concatenate([s.pos, np.array([[lo[0] + 0.3e-3, lo[1] + 0.3e-3, z0]])])
-    vel = np.concatenate([s.vel, np.array([[10.0, 10.0, -15.0]])])
-    G = s.n_grains
+def _append_node_grain(s, x, v, h, mat):
+    """Append a grain of one node at x with velocity v (a copy of the last
+    grain's gravity), volume h^dim and the material mat(3h)."""
+    g = _points_grain(np.zeros((1, s.dim)), h, s.dim, 3 * h, mat)
     s.node_offset = np.append(s.node_offset, s.node_offset[-1] + 1)
     s.conn_offset = np.append(s.conn_offset, s.conn_offset[-1])
     s.bdry_offset = np.append(s.bdry_offset, s.bdry_offset[-1] + 1)
     s.bdry_nodes = np.append(s.bdry_nodes, np.uint32(0))
     s.clamped_offset = np.append(s.clamped_offset, s.clamped_offset[-1])
-    s.pos, s.vel = pos, vel
-    s.vol = np.append(s.vol, h ** 3)
-    s.disp = np.zeros_like(pos)
-    s.acc = np.zeros_like(pos)
+    s.pos = np.concatenate([s.pos, np.asarray(x, float).reshape(1, -1)])
+    s.vel = np.concatenate([s.vel, np.asarray(v, float).reshape(1, -1)])
+    s.vol = np.append(s.vol, h ** s.dim)
+    s.disp = np.zeros_like(s.pos)
+    s.acc = np.zeros_like(s.pos)
     s.extforce = np.concatenate([s.extforce, s.extforce[-1:]])
-    for f, v in (("delta", g.delta), ("rho", g.rho), ("cnot", g.cnot), ("snot", g.snot)):
-        setattr(s, f, np.append(getattr(s, f), v))
+    for f, val in (("delta", g.delta), ("rho", g.rho), ("cnot", g.cnot), ("snot", g.snot)):
+        setattr(s, f, np.append(getattr(s, f), val))
     for f in ("movable", "breakable", "stoppable"):
         setattr(s, f, np.append(getattr(s, f), np.int32(1)))
     s.torque_axis = np.append(s.torque_axis, np.uint32(2))
     s.torque_val = np.append(s.torque_val, 0.0)
+
+
+def _one_node_3d(seed):
+    s, p = setups.compression_3d(seed=seed, n_side=2, R=0.5e-3, h_div=4, cube=3e-3,
+                                 wall_speed=-10.0)
+    lo = s.pos.min(axis=0)
+    # the node starts 1.5 contact radii above the bottom wall, under the
+    # lowest sphere, and falls onto the wall before rising into the spheres
+    z0 = s.wall[2] + 1.5 * s.contact_radius
+    G = s.n_grains
+    _append_node_grain(s, [lo[0] + 0.3e-3, lo[1] + 0.3e-3, z0], [10.0, 10.0, -15.0],
+                       0.5e-3 / 4, setups.material_3d)
     assert s.n_grains == G + 1
     return s, p
 
 
+def _free_point(s):
+    """The point inside the walls (4 contact radii in) farthest from every node."""
+    from scipy.spatial import cKDTree
+    d = s.dim
+    if d == 3:
+        lo, hi = np.array(s.wall[:3]), np.array(s.wall[3:6])
+    else:  # left, right, top, bottom
+        lo, hi = np.array([s.wall[0], s.wall[3]]), np.array([s.wall[1], s.wall[2]])
+    lo, hi = lo + 4 * s.contact_radius, hi - 4 * s.contact_radius
+    grid = np.stack(np.meshgrid(*[np.linspace(lo[k], hi[k], 25 if d == 3 else 200)
+                                  for k in range(d)], indexing="ij"), -1).reshape(-1, d)
+    dist, _ = cKDTree(s.pos).query(grid)
+    assert np.max(dist) > 4 * s.contact_radius
+    return grid[int(np.argmax(dist))]
+
+
+def _nice(x):
+    m = np.abs(x)
+    return (m == 0.0) | ((m >= 2.0 ** -240) & (m <= 2.0 ** 240))
+
+
+def _checked_2d(seed):
+    """The two-disk fracture with one node's y moved to 1e-80 (from within
+    0.05h of zero): a reference coordinate that is not nice, so every step
+    takes the bond loop with the per-bond range test and IEEE fallback."""
+    s, p = setups.two_disk_collision(seed=seed, gap=1e-5, speed=50.0)
+    nodes = s.grain_nodes(0)
+    s.vel[nodes] += 1e5 * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
+    k = int(np.argmin(np.abs(s.pos[:, 1])))
+    s.pos[k, 1] = 1e-80
+    assert not _nice(s.pos).all()
+    return s, p
+
+
+def _checked_3d(seed):
+    """The 8-sphere compression with one more one-node grain in the widest
+    free space, moving at 1e-75 m/s along x with no external force: its
+    displacement stays below 2^-240, so every step after the first takes the
+    range-tested bond loop."""
+    s, p = setups.compression_3d(seed=seed, n_side=2, R=0.5e-3, h_div=4, cube=3e-3,
+                                 wall_speed=-10.0)
+    _append_node_grain(s, _free_point(s), [1e-75, 0.0, 0.0], 0.5e-3 / 4, setups.material_3d)
+    s.extforce[-1] = 0.0
+    return s, p
+
+
+def _checked_2d_large(seed):
+    """The same free node in a 250-grain 2D settle (76k nodes, one lane per
+    node) with grains thrown at 60 m/s: the range-tested loop of the
+    large-workload kernel over a whole run."""
+    s, p = setups.bulk_settle_2d(seed=seed, n_grains=250, box=(40e-3, 80e-3),
+                                 nodes_per_grain=300)
+    rng = np.random.default_rng(seed)
+    for g in range(s.n_grains):
+        s.vel[s.grain_nodes(g)] += rng.normal(size=2) * 60.0
+    _append_node_grain(s, _free_point(s), [1e-75, 0.0], 1e-4, setups.material_2d)
+    s.extforce[-1] = 0.0
+    return s, p
+
+
 CASES = {"2d-one-node-loose-and-chain-grains": _mixed_2d, "2d-no-bonds": _no_bonds_2d,
-         "3d-one-node-grain": _one_node_3d}
+         "3d-one-node-grain": _one_node_3d, "2d-range-tested-loop": _checked_2d,
+         "3d-range-tested-loop": _checked_3d, "2d-range-tested-loop-76k": _checked_2d_large}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -112,10 +180,17 @@ def test_degenerate_grains_bit_exact(name, oracle_threads):
         for t in (1, 50, 200, 400):
             _advance(g, o, t - done)
             done = t
-            _assert_exact(g, o, f"{name} t={t}")
+            if s.dim == 2 and s.allow_wall:  # libm asin in the 2D wall force
+                _assert_close(g, o, 1e-10, f"{name} t={t}")
+            else:
+                _assert_exact(g, o, f"{name} t={t}")
         print(f"{name}: {len(o.contacts())} contact pairs, "
               f"{int((o.bond_intact() == 0).sum())} broken slots at t=400", flush=True)
         assert len(o.contacts()) > 0 or name == "2d-no-bonds"
+        if name in ("3d-range-tested-loop", "2d-range-tested-loop-76k"):
+            # the free node's x displacement is not nice
+            ux = o.read("disp")[-1, 0]
+            assert 0.0 < ux < 2.0 ** -240 and g.read("disp")[-1, 0] == ux
     finally:
         g.close()
         o.close()
