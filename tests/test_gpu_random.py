@@ -7,6 +7,8 @@ node).  Bit for bit, except 1e-10 of each field's largest value for 2D
 workloads with walls (the wall force calls libm asin, whose last bit may
 differ between host and device); bond status, contact pairs and wall
 positions exact in every case."""
+import os
+
 import numpy as np
 import pytest
 
@@ -16,7 +18,8 @@ from test_gpu_parity import _advance, _assert_close, _assert_exact, _both
 
 pytestmark = pytest.mark.gpu
 
-N_CASES = 20
+# PG_RANDOM_CASES widens the sweep (the draws are the same for the first 20)
+N_CASES = int(os.environ.get("PG_RANDOM_CASES", "20"))
 
 # switches that apply to every workload, and those for 3D only
 MODS = [opt._nl_bdry_only, opt._no_damping_no_friction, opt._no_friction, opt._no_self_contact,
