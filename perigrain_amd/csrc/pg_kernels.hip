@@ -16,8 +16,11 @@
 #include "pg_internal.h"
 #include "pg_kernels.h"
 
-// Waves per SIMD the step kernel is compiled for: 4 caps it at 128 VGPRs,
-// which the bond loop fits without spilling.
+// Waves per SIMD the one-lane step kernel is compiled for: 4 caps it at 128
+// VGPRs, the residency the bond loop's FP64 chains need.  The kernels with
+// several lanes per node run at most two wavefronts per SIMD (64k nodes
+// times two lanes, 16k times four), so they are compiled for two: up to 256
+// VGPRs, which the inputs they load up front take without spilling.
 #ifndef PG_STEP_WAVES_PER_EU
 #define PG_STEP_WAVES_PER_EU 4
 #endif
@@ -603,6 +606,32 @@ __device__ __forceinline__ void step_housekeeping(const StepArgs &A, uint32_t t,
   }
 }
 
+// A node's own step inputs that depend on its index alone: reference
+// position and displacement (record of step t), previous velocity,
+// acceleration, external force, position at the list build, flags, contact
+// key and its contact-list range.  None of them is written during the step
+// before its use (brk_new[i] only by node i's own bond loop, when `broke`
+// already says the same).
+template <int D> struct NodeIn {
+  double X[D], u[D], v[D], a[D], ex[D], xb[D];
+  uint32_t key, cl0, cl1, nfast;
+  uint8_t info, brk_new;
+  __device__ __forceinline__ void load(const StepArgs &A, uint32_t t, uint32_t i) {
+    rec_X<D>(A, i, X);
+    rec_u<D>(A.rec[t & 1], A.ns, i, u);
+    load_vec<D>(A.vel[(t + 1) & 1], A.ns, i, v);
+    load_vec<D>(A.acc, A.ns, i, a);
+    load_vec<D>(A.ext, A.ns, i, ex);
+    load_vec<D>(A.xb, A.ns, i, xb);
+    info = A.node_info[i];
+    key = A.ckey[i];
+    cl0 = A.cl_ptr[i];
+    cl1 = A.cl_ptr[i + 1];
+    nfast = A.cl_nfast[i];
+    brk_new = A.brk_new[i];
+  }
+};
+
 // The step of node i (all lanes of a wavefront call it together: the grain
 // boxes are combined across the wavefront).
 template <int D, bool RECORD, bool UVOL, int LPN = 1, class RS>
@@ -617,8 +646,16 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
   for (int k = 0; k < D; ++k) xn[k] = 0.0;
   bool viol = false;
 
+  // With several lanes per node (small workloads) a step lasts as long as
+  // one wavefront's chain of dependent loads, so there the node's inputs
+  // that depend on i alone are requested together, up front.  With one lane
+  // per node they are loaded where they are used, which leaves the
+  // registers to the bond loop.
+  constexpr bool EARLY = LPN > 1;
   if (valid) {
     g = A.node_grain[i];
+    NodeIn<D> early;
+    if constexpr (EARLY) early.load(A, t, i);
     const double *U = A.rec[t & 1];
     double F[D];
 #pragma unroll
@@ -638,8 +675,16 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
                          (iq.set_movable_grain == (int32_t)g && t >= iq.set_movable_step);
       if (mov_a) {
         double Xo[D], ui[D];
-        rec_X<D>(A, i, Xo);
-        rec_u<D>(U, A.ns, i, ui);
+        if constexpr (EARLY) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            Xo[k] = early.X[k];
+            ui[k] = early.u[k];
+          }
+        } else {
+          rec_X<D>(A, i, Xo);
+          rec_u<D>(U, A.ns, i, ui);
+        }
         const double cnot = gq->cnot, snot = gq->snot;
         const double gvol = UVOL ? gq->vol : 0.0;
         const bool brk = (iq.flags & IF_FRACTURE) && (gflags & GF_BREAKABLE);
@@ -669,18 +714,31 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     const GrainDev gp = A.grains[g];
     const InstDev &ip = A.insts[gp.inst];
     double Xc[D], ui[D], xc[D];
-    rec_u<D>(U, A.ns, i, ui);
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) ui[k] = early.u[k];
+    } else {
+      rec_u<D>(U, A.ns, i, ui);
+    }
     pos_X<D>(A, ip, g, i, Xc);
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = Xc[k] + ui[k]; // CurrPos = pos + disp
-    const uint8_t info = A.node_info[i];
+    const uint8_t info = EARLY ? early.info : A.node_info[i];
     const bool mov = grain_movable(gp, ip, g, t);
     const double *vprev = A.vel[(t + 1) & 1];
     double *vcur = A.vel[t & 1];
     double vi[D], ai[D];
-    load_vec<D>(vprev, ns, i, vi);
-    load_vec<D>(A.acc, ns, i, ai);
-    const uint32_t key = A.ckey[i];
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        vi[k] = early.v[k];
+        ai[k] = early.a[k];
+      }
+    } else {
+      load_vec<D>(vprev, ns, i, vi);
+      load_vec<D>(A.acc, ns, i, ai);
+    }
+    const uint32_t key = EARLY ? early.key : A.ckey[i];
     uint32_t wmask = 0; // bit s: the node touched wall side s
     double wt[6][4];
 #pragma unroll
@@ -699,9 +757,10 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
       if ((info & NI_CONTACT) && (selfc || stop)) {
         // the fast list while no bond of the node has broken since the list
         // build (its other entries are intact bonds, which stay excluded)
-        const uint32_t c0 = A.cl_ptr[i];
-        const bool full = broke || A.brk_new[i];
-        const uint32_t c1 = full ? A.cl_ptr[i + 1] : c0 + A.cl_nfast[i];
+        const uint32_t c0 = EARLY ? early.cl0 : A.cl_ptr[i];
+        const bool full = broke || (EARLY ? early.brk_new : A.brk_new[i]);
+        const uint32_t c1 = full ? (EARLY ? early.cl1 : A.cl_ptr[i + 1])
+                                 : c0 + (EARLY ? early.nfast : A.cl_nfast[i]);
         // the next fast-list entry is loaded while the current one is
         // evaluated, and a partner's data is loaded before the gates decide
         // whether it is needed (one round trip to memory per entry)
@@ -807,7 +866,12 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 
       // ---- external force (timeloop.h:776-786) and torque (:790-824)
       double ex[D];
-      load_vec<D>(A.ext, ns, i, ex);
+      if constexpr (EARLY) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) ex[k] = early.ex[k];
+      } else {
+        load_vec<D>(A.ext, ns, i, ex);
+      }
       if (ip.flags & IF_WORM) {
         // timeloop.h:1183: extforce times (t_num + 1) % 2, i.e. only in the
         // phases with the front nodes clamped (adding 0*extforce leaves F)
@@ -903,7 +967,7 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
     if (info & NI_CONTACT) {
       double dd[D];
 #pragma unroll
-      for (int k = 0; k < D; ++k) dd[k] = xn[k] - A.xb[(size_t)k * ns + i];
+      for (int k = 0; k < D; ++k) dd[k] = xn[k] - (EARLY ? early.xb[k] : A.xb[(size_t)k * ns + i]);
       viol = red<D>(dd, dd) > ip.skin_half2;
     }
     if (RECORD) {
@@ -927,8 +991,8 @@ __device__ __forceinline__ void step_node(const StepArgs &A, const uint32_t t, c
 // its bond loop (bond_loop_lanes) and repeat the rest of its step, which
 // writes the same values and whose atomics (min, max, or) are idempotent.
 template <int D, bool RECORD, bool UVOL, int LPN>
-__global__ __launch_bounds__(256, PG_STEP_WAVES_PER_EU) void k_step(const StepArgs A,
-                                                                   const uint32_t t_arg) {
+__global__ __launch_bounds__(256, LPN > 1 ? 2 : PG_STEP_WAVES_PER_EU) void k_step(
+    const StepArgs A, const uint32_t t_arg) {
   const uint32_t t = step_index(A, t_arg);
   if (*A.violation <= t) return; // neighbour list stale since an earlier step
   const uint32_t nb = gridDim.x;
