@@ -1,6 +1,7 @@
 """Randomised GPU parity: seeded draws of a workload, grain velocities, an
 optional expanding grain (bond breaks), and up to three of the configuration
-switches of test_gpu_options at once, each run against the oracle.  The
+switches of test_gpu_options at once, each run against the oracle, alone and
+as instances of batched handles.  The
 workloads span every step-kernel form: small handles (four lanes per node),
 a 2D handle of about 30k nodes (two lanes) and one of about 75k (one lane per
 node).  Bit for bit, except 1e-10 of each field's largest value for 2D
@@ -12,9 +13,10 @@ import os
 import numpy as np
 import pytest
 
+import oracle
 import test_gpu_options as opt
-from perigrain_amd import setups
-from test_gpu_parity import _advance, _assert_close, _assert_exact, _both
+from perigrain_amd import Stepper, setups
+from test_gpu_parity import FIELDS, _advance, _assert_close, _assert_exact, _both
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +63,84 @@ def _draw(case):
         pool[k](s, p)
         names.append(pool[k].__name__.strip("_"))
     return kind, s, p, names
+
+
+def _draw_small(rng, dim):
+    """A small workload of the given dimension with random velocities, maybe
+    an expanding grain, and 0-3 configuration switches (as _draw)."""
+    seed = int(rng.integers(0, 1 << 30))
+    if dim == 3:
+        s, p = setups.compression_3d(seed=seed, n_side=2, R=0.5e-3, h_div=int(rng.integers(4, 6)),
+                                     cube=3e-3, wall_speed=-float(rng.uniform(5.0, 30.0)))
+    elif rng.uniform() < 0.5:
+        s, p = setups.bulk_settle_2d(seed=seed, n_grains=int(rng.integers(4, 13)),
+                                     box=(8e-3, 16e-3), nodes_per_grain=int(rng.integers(40, 160)))
+    else:
+        s, p = setups.two_disk_collision(seed=seed, gap=float(rng.uniform(1e-5, 6e-5)),
+                                         speed=float(rng.uniform(10.0, 60.0)))
+    for g in range(s.n_grains):
+        s.vel[s.grain_nodes(g)] += rng.normal(size=s.dim) * float(rng.uniform(20.0, 150.0))
+    names = []
+    if rng.uniform() < 0.5:
+        g = int(rng.integers(0, s.n_grains))
+        nodes = s.grain_nodes(g)
+        s.vel[nodes] += float(rng.uniform(3e4, 1e5)) * (s.pos[nodes] - s.pos[nodes].mean(axis=0))
+        names.append(f"expand-{g}")
+    pool = MODS + (MODS_3D if dim == 3 else [])
+    for k in rng.choice(len(pool), size=int(rng.integers(0, 4)), replace=False):
+        pool[k](s, p)
+        names.append(pool[k].__name__.strip("_"))
+    return s, p, names
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_random_batched_instances_match_the_oracle(case, oracle_threads):
+    """Three to five random small instances of one dimension, each with its
+    own switches, time step and walls, in one batched handle: every instance
+    against the oracle's run of it (bit for bit; 1e-10 for 2D instances with
+    walls)."""
+    rng = np.random.default_rng(5000 + case)
+    dim = 3 if case % 3 == 2 else 2
+    draws = [_draw_small(rng, dim) for _ in range(int(rng.integers(3, 6)))]
+    what = f"batch {case}: " + ", ".join(f"{s.n_nodes} nodes {'+'.join(n) or 'plain'}"
+                                          for s, _, n in draws)
+    print(what, flush=True)
+    os_ = [oracle.Oracle(s, p, threads=oracle_threads) for s, p, _ in draws]
+    try:
+        with Stepper([s for s, _, _ in draws], [p for _, p, _ in draws], device=0) as b:
+            done = 0
+            for t in (1, 60, 150):
+                b.step(t - done)
+                for k, (o, (s, _, _)) in enumerate(zip(os_, draws)):
+                    o.step(t - done, record_contacts=True)
+                    close = s.dim == 2 and s.allow_wall
+                    for f in FIELDS:
+                        a, c = b.read(f, inst=k), o.read(f)
+                        if close:
+                            err = np.max(np.abs(a - c)) / max(np.max(np.abs(c)), 1e-300)
+                            assert err <= 1e-10, f"instance {k} t={t} {f}: {err:.3e}"
+                        else:
+                            assert np.array_equal(a, c), \
+                                f"instance {k} t={t} {f}: {int((a != c).sum())} entries differ"
+                    assert np.array_equal(b.bond_intact(inst=k), o.bond_intact()), \
+                        f"instance {k} t={t} bond status"
+                    assert np.array_equal(b.contacts(inst=k), o.contacts()), \
+                        f"instance {k} t={t} contact pairs"
+                    wb, rb = b.wall(inst=k)
+                    wo, ro = o.wall()
+                    assert np.array_equal(wb, wo), f"instance {k} t={t} wall positions"
+                    if close:
+                        scale = max(np.max(np.abs(ro)), 1e-300)
+                        assert np.max(np.abs(rb - ro)) / scale <= 1e-10, f"instance {k} t={t} reaction"
+                    else:
+                        assert np.array_equal(rb, ro), f"instance {k} t={t} wall reaction"
+                done = t
+        print(f"batch {case}: " + ", ".join(
+            f"{len(o.contacts())} pairs/{int((o.bond_intact() == 0).sum())} broken" for o in os_),
+            flush=True)
+    finally:
+        for o in os_:
+            o.close()
 
 
 @pytest.mark.parametrize("case", range(N_CASES))
