@@ -169,3 +169,72 @@ def test_adjoint_with_bonds_breaking_mid_segment(oracle_threads):
     raw_cpu = np.einsum("tsk,tskq->q", bar_c, dR)
     print("adjoint", raw, "cpu dual numbers", raw_cpu, flush=True)
     np.testing.assert_allclose(raw, raw_cpu, rtol=1e-8)
+
+
+def _random_adjoint_case(case):
+    """A small 2D or 3D compression with a random wall speed, length and
+    checkpoint interval, maybe fracture with the top grain expanding, and up
+    to three of the switches the differentiable form supports."""
+    import test_gpu_options as opt
+    from perigrain_amd import setups
+    mods = [opt._no_damping_no_friction, opt._no_friction, opt._no_self_contact,
+            opt._nl_bdry_only, opt._nonuniform_volume, opt._bdry_set_order,
+            opt._extforce_ramp, opt._grain0_not_breakable]
+    rng = np.random.default_rng(7000 + case)
+    seed = int(rng.integers(0, 1 << 30))
+    if case % 3 == 2:
+        s, p = setups.compression_3d(seed=seed, n_side=2, R=0.5e-3, h_div=4, cube=3e-3,
+                                     wall_speed=-float(rng.uniform(20.0, 60.0)))
+    else:
+        s, p = setups.compression_2d_diff(seed=seed, n_grains=int(rng.integers(3, 7)),
+                                          nodes_per_grain=int(rng.integers(40, 90)),
+                                          wall_speed=-float(rng.uniform(5.0, 20.0)))
+    names = []
+    if rng.uniform() < 0.5:
+        p.enable_fracture = 1
+        g = int(np.argmax([s.pos[s.grain_nodes(k), -1].max() for k in range(s.n_grains)]))
+        nd = s.grain_nodes(g)
+        r = s.pos[nd] - s.pos[nd].mean(axis=0)
+        spread = 1.0 + 0.5 * rng.standard_normal((r.shape[0], 1))
+        s.vel[nd] += float(rng.uniform(2e4, 5e4)) * r * spread
+        names.append(f"fracture-expand-{g}")
+    for k in rng.choice(len(mods), size=int(rng.integers(0, 4)), replace=False):
+        mods[k](s, p)
+        names.append(mods[k].__name__.strip("_"))
+    steps, ck = int(rng.integers(60, 140)), int(rng.integers(15, 50))
+    return s, p, steps, ck, names, rng
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_random_adjoint_cases_match_cpu_dual_numbers(case, oracle_threads):
+    # the gradient of a random weighting of every wall's reaction history,
+    # against the oracle on dual numbers (the same frozen-branch map)
+    import oracle
+    from perigrain_amd import Stepper
+    s, p, steps, ck, names, rng = _random_adjoint_case(case)
+    print(f"case {case}: {s.dim}D, {s.n_nodes} nodes, {steps} steps, checkpoints every {ck}, "
+          f"{'+'.join(names) or 'plain'}", flush=True)
+    R, dR = oracle.reaction_history_tangent(s, p, steps, threads=oracle_threads)
+    top = 2 if s.dim == 2 else 5
+    with Stepper(s, p, device=0) as g:
+        hist = g.adjoint_forward(steps, ck)
+        if s.dim == 3:
+            assert np.array_equal(hist, R), "the recorded 3D reaction history is bit-exact"
+        else:
+            np.testing.assert_allclose(hist, R, rtol=1e-10, atol=1e-10 * np.abs(R).max())
+        assert np.any(hist[:, top] != 0), "the top wall should push"
+        x0, v0, b0 = g.read("CurrPos"), g.read("vel"), g.bond_intact()
+        if p.enable_fracture:
+            assert np.any(b0 == 0), "the expanding grain should break bonds"
+        bar = rng.normal(size=hist.shape) * np.abs(hist).max()
+        raw = g.adjoint_backward(bar)
+        assert np.array_equal(g.read("CurrPos"), x0) and np.array_equal(g.read("vel"), v0)
+        assert np.array_equal(g.bond_intact(), b0)
+    raw_cpu = np.einsum("tsk,tskq->q", bar, dR)
+    # each parameter's tolerance scales with the sum of its terms' magnitudes,
+    # so a sum that cancels, or a parameter the run does not depend on (μ
+    # with friction off: zero on both sides), is still held to 1e-8
+    mag = np.einsum("tsk,tskq->q", np.abs(bar), np.abs(dR))
+    print("adjoint", raw, "cpu dual numbers", raw_cpu, "term magnitudes", mag,
+          f"{int((b0 == 0).sum())} broken slots", flush=True)
+    assert np.all(np.abs(raw - raw_cpu) <= 1e-8 * mag), (raw, raw_cpu, mag)
