@@ -153,13 +153,15 @@ def batched_bench(steps, seed, n_instances=256):
     }
 
 
-def adjoint_bench(steps=2000, ck=100):
+def adjoint_bench(steps=2000, ck_segments=100):
     from perigrain_amd import Stepper, diffrun
-    run = diffrun.compression_2d_run(steps=steps, checkpoint_every=ck)
+    run = diffrun.compression_2d_run(steps=steps)
     theta0, theta_t = (1.5e9, 1.0, 0.25), (1.0e9, 1.2, 0.4)
     target, st = run.forward(*theta_t)
     st[0].close()
     s, p = run.make_setup(*theta0)
+    # every step's state kept (about 1.4 GB here), no recompute
+    ck = diffrun.auto_checkpoint_every(s, steps)
     times = {}
     for rep in range(2):  # the first repetition warms up
         with Stepper(s, p, device=0) as g:
@@ -177,8 +179,22 @@ def adjoint_bench(steps=2000, ck=100):
             t0 = time.perf_counter()
             raw = g.adjoint_backward(bar)
             t_bwd = time.perf_counter() - t0
+        # the same with checkpoints every ck_segments steps, each segment
+        # recomputed in the backward pass
+        with Stepper(s, p, device=0) as g:
+            g.sync()
+            t0 = time.perf_counter()
+            hist_c = g.adjoint_forward(steps, ck_segments)
+            t_rec_c = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            raw_seg = g.adjoint_backward(diffrun.top_wall_misfit(hist_c, target)[1])
+            t_bwd_c = time.perf_counter() - t0
         times = {"forward_ms": t_fwd * 1e3, "recorded_forward_ms": t_rec * 1e3,
                  "backward_ms": t_bwd * 1e3}
+        checkpointed = {"checkpoint_every": ck_segments, "recorded_forward_ms": t_rec_c * 1e3,
+                        "backward_ms": t_bwd_c * 1e3,
+                        "gradient_rel_diff": [abs(a - b) / abs(b) if b else abs(a)
+                                              for a, b in zip(raw_seg, raw)]}
     Kn = s.normal_stiffness
     grad = [(raw[0] + Kn * raw[1]) / theta0[0], Kn * raw[1] / theta0[1], raw[2]]
     fd = []
@@ -207,7 +223,9 @@ def adjoint_bench(steps=2000, ck=100):
     grad_c = [(raw_c[0] + Kn * raw_c[1]) / theta0[0], Kn * raw_c[1] / theta0[1], raw_c[2]]
     rel_c = [abs(a - b) / abs(b) for a, b in zip(grad, grad_c)]
     print(f"bench: adjoint forward {times['forward_ms']:.1f} ms, recorded "
-          f"{times['recorded_forward_ms']:.1f} ms, backward {times['backward_ms']:.1f} ms, "
+          f"{times['recorded_forward_ms']:.1f} ms, backward {times['backward_ms']:.1f} ms "
+          f"(checkpoints every {ck_segments}: recorded {checkpointed['recorded_forward_ms']:.1f} ms, "
+          f"backward {checkpointed['backward_ms']:.1f} ms), "
           f"FD rel err {rel}, CPU dual-number rel err {rel_c} ({t_cpu:.1f} s)",
           file=sys.stderr, flush=True)
     return {
@@ -216,6 +234,7 @@ def adjoint_bench(steps=2000, ck=100):
         "steps": steps,
         "checkpoint_every": ck,
         **times,
+        "checkpointed": checkpointed,
         "adjoint_over_forward": times["backward_ms"] / times["forward_ms"],
         "recorded_plus_adjoint_over_forward":
             (times["recorded_forward_ms"] + times["backward_ms"]) / times["forward_ms"],

@@ -307,6 +307,9 @@ int pg_set_graph(pg_handle *h, int use_graph);
  *   grad[2]: the friction coefficient.
  * Bond status, contact pairs and every branch are frozen to the forward
  * pass; segments between checkpoints are recomputed from the checkpoints.
+ * With checkpoint_every >= nsteps (one segment) the forward pass keeps every
+ * step's state instead (node records, velocities, walls and boxes: about
+ * 24*dim bytes per node and step) and the backward pass recomputes nothing.
  * Requires no torque, no timed toggles and no clamped nodes.  After the
  * backward pass the handle is back in the state after the recorded steps. */
 int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,

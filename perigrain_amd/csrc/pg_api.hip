@@ -174,6 +174,10 @@ struct pg_handle {
   size_t stats_cap = 0;
   struct Adj {
     bool recorded = false;
+    // the recorded pass kept every step's state in the segment slots (one
+    // segment: checkpoint_every >= nsteps), so the backward pass recomputes
+    // nothing
+    bool stored = false;
     uint32_t t0 = 0, T = 0, K = 0, nck = 0;
     uint32_t *break_step = nullptr; // [E + tail] step at which each slot's bond broke
     double *hist = nullptr;         // [T][18]
@@ -1139,7 +1143,95 @@ uint32_t next_reset(const pg_handle *h) {
   return r;
 }
 
-int run_steps(pg_handle *h, uint32_t nsteps) {
+template <class T> int d2d(pg_handle *h, T *dst, const T *src, size_t n) {
+  if (n == 0) return PG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+  return PG_OK;
+}
+
+// Step-indexed state of steps c+1 .. c+n of the differentiable form: what
+// step s reads (u^s with X, v^{s-1}, the wall positions and grain boxes of
+// step s) lives in slot s-c-1 of the segment buffers, and each step writes
+// its results straight into the next slot, so keeping them costs no copies.
+// Past the segment the handle's own buffers take the writes.  The slots' X
+// parts must hold X (fill_slot_X).
+struct SegMap {
+  uint32_t c, n;
+  bool in(uint32_t s) const { return s > c && s - c - 1 < n; }
+  double *R(pg_handle *h, uint32_t s) const {
+    return in(s) ? h->adj.su + (size_t)(s - c - 1) * rec_len(h) : h->rec[s & 1];
+  }
+  double *V(pg_handle *h, uint32_t s) const {
+    return in(s) ? h->adj.sv + (size_t)(s - c - 1) * vec_len(h) : h->vel[(s + 1) & 1];
+  }
+  double *W(pg_handle *h, uint32_t s) const {
+    return in(s) ? h->adj.swall + (size_t)(s - c - 1) * h->I * 6 : h->wallpos[s & 1];
+  }
+  unsigned long long *B(pg_handle *h, uint32_t s) const {
+    return in(s) ? h->adj.sbox + (size_t)(s - c - 1) * box_len(h) : h->bbox[s % 3];
+  }
+  // the launch arguments of step t
+  void remap(pg_handle *h, uint32_t t, StepArgs &As) const {
+    As.rec[t & 1] = R(h, t);
+    As.rec[(t + 1) & 1] = R(h, t + 1);
+    As.vel[(t + 1) & 1] = V(h, t);
+    As.vel[t & 1] = V(h, t + 1);
+    As.wallpos[t & 1] = W(h, t);
+    As.wallpos[(t + 1) & 1] = W(h, t + 1);
+    As.bbox[t % 3] = B(h, t);
+    As.bbox[(t + 1) % 3] = B(h, t + 1);
+    As.bbox[(t + 2) % 3] = B(h, t + 2);
+  }
+  // slot 0 from the handle's state after step c (h->t == c), and the cleared
+  // box buffer that step c+1 fills for step c+2
+  int seed(pg_handle *h) const {
+    CopySet cs;
+    auto w8 = [](const void *p) { return (unsigned long long *)const_cast<void *>(p); };
+    cs.dst[0] = w8(R(h, c + 1));
+    cs.src[0] = w8(h->rec[(c + 1) & 1]);
+    cs.n[0] = rec_len(h);
+    cs.dst[1] = w8(V(h, c + 1));
+    cs.src[1] = w8(h->vel[c & 1]);
+    cs.n[1] = vec_len(h);
+    cs.dst[2] = w8(W(h, c + 1));
+    cs.src[2] = w8(h->wallpos[(c + 1) & 1]);
+    cs.n[2] = (size_t)h->I * 6;
+    cs.dst[3] = w8(B(h, c + 1));
+    cs.src[3] = w8(h->bbox[(c + 1) % 3]);
+    cs.n[3] = box_len(h);
+    HIPCHK(launch_copy4(cs, h->stream));
+    if (B(h, c + 2) != h->bbox[(c + 2) % 3]) RC(d2d(h, B(h, c + 2), h->bbox[(c + 2) % 3], box_len(h)));
+    return PG_OK;
+  }
+  // after steps up to c+n ran on the slots: the handle's buffers of step c+n
+  // (u^{c+n}, v^{c+n-1}, its walls and boxes) from the last slot, so that
+  // the handle reads and steps on as after unmapped steps
+  int settle(pg_handle *h) const {
+    const uint32_t s = c + n;
+    RC(d2d(h, h->rec[s & 1], R(h, s), rec_len(h)));
+    RC(d2d(h, h->vel[(s + 1) & 1], V(h, s), vec_len(h)));
+    RC(d2d(h, h->wallpos[s & 1], W(h, s), (size_t)h->I * 6));
+    RC(d2d(h, h->bbox[s % 3], B(h, s), box_len(h)));
+    return PG_OK;
+  }
+};
+
+// The X parts of the first n record slots (steps write only u): every slot
+// takes the whole record of rec[0], by copies that double the filled range.
+int fill_slot_X(pg_handle *h, uint32_t n) {
+  if (n == 0) return PG_OK;
+  RC(d2d(h, h->adj.su, h->rec[0], rec_len(h)));
+  for (uint32_t done = 1; done < n;) {
+    const uint32_t m = std::min(done, n - done);
+    RC(d2d(h, h->adj.su + (size_t)done * rec_len(h), h->adj.su, (size_t)m * rec_len(h)));
+    done += m;
+  }
+  return PG_OK;
+}
+
+// Steps h->t+1 .. h->t+nsteps.  map: the steps of a segment of the
+// differentiable form read and write its slots (no graph replay then).
+int run_steps(pg_handle *h, uint32_t nsteps, const SegMap *map = nullptr) {
   if (nsteps == 0) return PG_OK;
   const uint32_t t_end = h->t + nsteps;
   if (t_end < h->t || t_end >= 0xFFFFFFF0u) return fail(PG_ERR_INVALID, "step counter overflow");
@@ -1160,7 +1252,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
     if (tr > h->t + 1) chunk = std::min<uint32_t>(chunk, tr - 1 - h->t);
     // a whole chunk without the recording step can be replayed from a graph
     const bool replay =
-        h->use_graph && !h->hist && chunk == h->check_every && h->t + chunk < t_end;
+        !map && h->use_graph && !h->hist && chunk == h->check_every && h->t + chunk < t_end;
     if (replay) {
       RC(ensure_graph(h));
       h->h_pin[8] = h->t; // read by the copy below, before the chunk's final sync
@@ -1172,15 +1264,16 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else {
       for (uint32_t s = h->t + 1; s <= h->t + chunk; ++s) {
-        if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, h->A, s, h->gmean, h->stream));
+        StepArgs As = h->A;
+        if (map) map->remap(h, s, As);
+        if (h->any_torque) HIPCHK(launch_grain_mean(h->dim, As, s, h->gmean, h->stream));
         if (h->hist) { // recording: the step's wall terms go to its ring slot
-          StepArgs As = h->A;
           const uint32_t slot = (s - 1) % h->check_every;
           As.wterm = h->wring + (size_t)slot * wring_stride(h);
           As.wflag = h->fring + (size_t)slot * fring_stride(h);
           HIPCHK(launch_step(h->dim, true, As, s, h->block, h->stream));
         } else {
-          HIPCHK(launch_step(h->dim, s == t_end, h->A, s, h->block, h->stream));
+          HIPCHK(launch_step(h->dim, s == t_end, As, s, h->block, h->stream));
         }
       }
     }
@@ -1213,7 +1306,7 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
       h->skipped += (uint64_t)(h->t + chunk) - v + 1;
       h->t = v - 1;
       react_queued = false;
-      RC(build_list(h, v));
+      RC(build_list(h, v, map ? map->R(h, v) : nullptr));
       continue;
     }
     h->t += chunk;
@@ -1229,12 +1322,6 @@ int run_steps(pg_handle *h, uint32_t nsteps) {
 
 // ------------------------------------------------------------ adjoint
 
-
-template <class T> int d2d(pg_handle *h, T *dst, const T *src, size_t n) {
-  if (n == 0) return PG_OK;
-  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
-  return PG_OK;
-}
 
 // Save the state after step t (everything step t+1 reads) as checkpoint k.
 int ck_save(pg_handle *h, uint32_t k, uint32_t t) {
@@ -1285,12 +1372,13 @@ int adj_alloc(pg_handle *h, uint32_t T, uint32_t K) {
     RC(dalloc(h, a.ck_wall, a.cap_ck * h->I * 6));
     RC(dalloc(h, a.ck_box, a.cap_ck * box_len(h)));
   }
-  if (K > a.cap_seg) {
+  const uint32_t seg = std::min(K, T); // slots of one segment
+  if (seg > a.cap_seg) {
     RC(dfree(h, a.su));
     RC(dfree(h, a.sv));
     RC(dfree(h, a.swall));
     RC(dfree(h, a.sbox));
-    a.cap_seg = K;
+    a.cap_seg = seg;
     RC(dalloc(h, a.su, a.cap_seg * rec_len(h)));
     RC(dalloc(h, a.sv, a.cap_seg * vec_len(h)));
     RC(dalloc(h, a.swall, a.cap_seg * h->I * 6));
@@ -1320,63 +1408,17 @@ int adj_alloc(pg_handle *h, uint32_t T, uint32_t K) {
   return PG_OK;
 }
 
-// Forward recompute of steps c+1 .. c+n from checkpoint state, saving before
-// every step t what the adjoint of step t reads: u^t, v^{t-1}, the wall
-// positions and the grain boxes of step t.
+// Forward recompute of steps c+1 .. c+n from checkpoint state, keeping in
+// the segment slots what the adjoint of each step t reads: u^t, v^{t-1}, the
+// wall positions and the grain boxes of step t.
 int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
-  auto &a = h->adj;
   const uint32_t end = c + n;
-  // What step s of the segment reads (u^s with X, v^{s-1}, walls, boxes)
-  // lives in slot s-c-1 of the segment buffers, and each step writes its
-  // results straight into the next slot, so the snapshots cost no copies.
-  // Past the segment the handle's own buffers take the writes (they are
-  // scratch until the next checkpoint restore).  The slots' X parts were
-  // filled once by pg_adjoint_backward.
-  auto seg = [&](uint32_t s) { return s > c && s - c - 1 < n; };
-  auto Rs = [&](uint32_t s) {
-    return seg(s) ? a.su + (size_t)(s - c - 1) * rec_len(h) : h->rec[s & 1];
-  };
-  auto Vs = [&](uint32_t s) {
-    return seg(s) ? a.sv + (size_t)(s - c - 1) * vec_len(h) : h->vel[(s + 1) & 1];
-  };
-  auto Ws = [&](uint32_t s) {
-    return seg(s) ? a.swall + (size_t)(s - c - 1) * h->I * 6 : h->wallpos[s & 1];
-  };
-  auto Bs = [&](uint32_t s) {
-    return seg(s) ? a.sbox + (size_t)(s - c - 1) * box_len(h) : h->bbox[s % 3];
-  };
-  // slot 0: the restored state after step c; the boxes step c+1 collects
-  // start from the cleared buffer ck_restore left for them
-  {
-    CopySet cs;
-    auto w8 = [](const void *p) { return (unsigned long long *)const_cast<void *>(p); };
-    cs.dst[0] = w8(Rs(c + 1));
-    cs.src[0] = w8(h->rec[(c + 1) & 1]);
-    cs.n[0] = rec_len(h);
-    cs.dst[1] = w8(Vs(c + 1));
-    cs.src[1] = w8(h->vel[c & 1]);
-    cs.n[1] = vec_len(h);
-    cs.dst[2] = w8(Ws(c + 1));
-    cs.src[2] = w8(h->wallpos[(c + 1) & 1]);
-    cs.n[2] = (size_t)h->I * 6;
-    cs.dst[3] = w8(Bs(c + 1));
-    cs.src[3] = w8(h->bbox[(c + 1) % 3]);
-    cs.n[3] = box_len(h);
-    HIPCHK(launch_copy4(cs, h->stream));
-    if (Bs(c + 2) != h->bbox[(c + 2) % 3]) RC(d2d(h, Bs(c + 2), h->bbox[(c + 2) % 3], box_len(h)));
-  }
+  const SegMap m{c, n};
+  RC(m.seed(h));
   while (h->t < end) {
     for (uint32_t t = h->t + 1; t <= end; ++t) {
       StepArgs As = h->A;
-      As.rec[t & 1] = Rs(t);
-      As.rec[(t + 1) & 1] = Rs(t + 1);
-      As.vel[(t + 1) & 1] = Vs(t);
-      As.vel[t & 1] = Vs(t + 1);
-      As.wallpos[t & 1] = Ws(t);
-      As.wallpos[(t + 1) & 1] = Ws(t + 1);
-      As.bbox[t % 3] = Bs(t);
-      As.bbox[(t + 1) % 3] = Bs(t + 1);
-      As.bbox[(t + 2) % 3] = Bs(t + 2);
+      m.remap(h, t, As);
       HIPCHK(launch_step(h->dim, false, As, t, h->block, h->stream));
     }
     HIPCHK(hipMemcpyAsync(h->h_pin, h->violation, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1386,7 +1428,7 @@ int adj_recompute(pg_handle *h, uint32_t c, uint32_t n) {
     if (v <= end) {
       if (v <= h->t) return fail(PG_ERR_STATE, "neighbour list invalid in the recompute");
       h->t = v - 1;
-      RC(build_list(h, v, Rs(v)));
+      RC(build_list(h, v, m.R(h, v)));
       continue;
     }
     h->t = end;
@@ -1859,14 +1901,27 @@ int pg_adjoint_forward(pg_handle *h, uint32_t nsteps, uint32_t checkpoint_every,
     a.T = nsteps;
     a.K = checkpoint_every;
     a.recorded = false;
+    a.stored = false;
     RC(ck_save(h, 0, h->t));
     h->hist = a.hist;
     h->hist_t0 = a.t0;
     int rc = PG_OK;
-    for (uint32_t k = 0; k < a.nck && rc == PG_OK; ++k) {
-      const uint32_t n = std::min(a.K, a.T - k * a.K);
-      rc = run_steps(h, n);
-      if (rc == PG_OK) rc = ck_save(h, k + 1, h->t);
+    if (a.nck == 1) {
+      // one segment: the steps run on its slots, which then hold every
+      // step's state for the backward pass
+      const SegMap m{a.t0, a.T};
+      rc = fill_slot_X(h, a.T);
+      if (rc == PG_OK) rc = m.seed(h);
+      if (rc == PG_OK) rc = run_steps(h, a.T, &m);
+      if (rc == PG_OK) rc = m.settle(h);
+      if (rc == PG_OK) rc = ck_save(h, 1, h->t);
+      a.stored = rc == PG_OK;
+    } else {
+      for (uint32_t k = 0; k < a.nck && rc == PG_OK; ++k) {
+        const uint32_t n = std::min(a.K, a.T - k * a.K);
+        rc = run_steps(h, n);
+        if (rc == PG_OK) rc = ck_save(h, k + 1, h->t);
+      }
     }
     h->hist = nullptr;
     RC(rc);
@@ -1902,16 +1957,18 @@ int pg_adjoint_backward(pg_handle *h, const double *reaction_bar, double *grad) 
     const uint32_t blocks = (uint32_t)(((size_t)h->N * ADJ_LANES + 255) / 256);
     HIPCHK(hipMemsetAsync(a.gpart, 0, (size_t)blocks * 3 * sizeof(double), h->stream));
     // X parts of the segment's record slots (the recompute writes only u)
-    for (uint32_t q = 0; q < std::min(a.K, a.T); ++q) RC(d2d(h, a.su + (size_t)q * rec_len(h), h->rec[0], rec_len(h)));
+    if (!a.stored) RC(fill_slot_X(h, std::min(a.K, a.T)));
     InstHost &ih = h->inst[0];
     const double skin0 = ih.skin;
     int rc = PG_OK;
     for (int64_t k = (int64_t)a.nck - 1; k >= 0 && rc == PG_OK; --k) {
       const uint32_t c = a.t0 + (uint32_t)k * a.K;
       const uint32_t n = std::min(a.K, a.T - (uint32_t)k * a.K);
-      rc = ck_restore(h, (uint32_t)k, c);
-      if (rc == PG_OK) rc = adj_recompute(h, c, n);
-      if (rc != PG_OK) break;
+      if (!a.stored) { // the segment's states: from its checkpoint, recomputed
+        rc = ck_restore(h, (uint32_t)k, c);
+        if (rc == PG_OK) rc = adj_recompute(h, c, n);
+        if (rc != PG_OK) break;
+      }
       // one candidate list for the whole segment: pairs within the contact
       // radius at any of its steps are within radius + twice the largest
       // displacement at its first step

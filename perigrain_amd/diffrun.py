@@ -18,19 +18,39 @@ from . import setups
 from .stepper import Stepper
 
 
-class DiffRun:
-    """Forward and backward runs of one problem family on a device."""
+def trajectory_bytes(s, steps):
+    """Device bytes that keeping every step's state of a recorded pass takes:
+    per step the node records (X and u), velocities, grain boxes and walls."""
+    ns = (s.n_nodes + 1 + 63) // 64 * 64  # nodes and the ghost record, padded
+    return steps * (ns * 3 * s.dim * 8 + s.n_grains * 2 * s.dim * 8 + 6 * 8)
 
-    def __init__(self, make_setup, steps, checkpoint_every=100, device=0):
+
+def auto_checkpoint_every(s, steps, budget=16 << 30):
+    """checkpoint_every for pg_adjoint_forward: the whole run, so that the
+    backward pass reuses every step's state instead of recomputing it, when
+    that state fits in `budget` bytes of device memory; otherwise
+    checkpoints every ceil(sqrt(steps)) steps (each segment recomputed once
+    in the backward pass)."""
+    if trajectory_bytes(s, steps) <= budget:
+        return int(steps)
+    return int(np.ceil(np.sqrt(steps)))
+
+
+class DiffRun:
+    """Forward and backward runs of one problem family on a device.
+    checkpoint_every None: auto_checkpoint_every."""
+
+    def __init__(self, make_setup, steps, checkpoint_every=None, device=0):
         self.make_setup = make_setup  # (K, scale, mu) -> (Setup, Params)
         self.steps = int(steps)
-        self.checkpoint_every = int(checkpoint_every)
+        self.checkpoint_every = None if checkpoint_every is None else int(checkpoint_every)
         self.device = device
 
     def forward(self, K, scale, mu):
         s, p = self.make_setup(K, scale, mu)
+        ck = self.checkpoint_every or auto_checkpoint_every(s, self.steps)
         g = Stepper(s, p, device=self.device)
-        hist = g.adjoint_forward(self.steps, self.checkpoint_every)
+        hist = g.adjoint_forward(self.steps, ck)
         return hist, (g, K, scale, mu, s.normal_stiffness)
 
     def plain(self, K, scale, mu):
@@ -65,7 +85,7 @@ def top_wall_misfit(hist, target, side=2):
     return float((d ** 2).sum()), bar
 
 
-def compression_2d_run(steps=2000, checkpoint_every=100, device=0, **kw):
+def compression_2d_run(steps=2000, checkpoint_every=None, device=0, **kw):
     """The differentiable 2D compression (64 disks and squares, ~250 nodes
     each, top wall at -0.5 m/s, dt = 2e-8) as a DiffRun."""
     def make(K, scale, mu):

@@ -238,3 +238,41 @@ def test_random_adjoint_cases_match_cpu_dual_numbers(case, oracle_threads):
     print("adjoint", raw, "cpu dual numbers", raw_cpu, "term magnitudes", mag,
           f"{int((b0 == 0).sum())} broken slots", flush=True)
     assert np.all(np.abs(raw - raw_cpu) <= 1e-8 * mag), (raw, raw_cpu, mag)
+
+
+def test_stored_trajectory_equals_checkpointed_recompute():
+    # One segment (checkpoint_every >= nsteps) keeps every step's state from
+    # the recorded pass; several segments recompute them from checkpoints.
+    # Both give the same reaction history and gradient (the latter to the
+    # order in which the contact terms of different candidate lists are
+    # summed), and the handle reads and steps on like a plain run.
+    from perigrain_amd import Stepper
+    s, p = _fracturing_compression()
+    steps = 120
+    fields = ("CurrPos", "vel", "acc", "force")
+    with Stepper(s, p, device=0) as ref:
+        ref.step(steps)
+        want = {f: ref.read(f) for f in fields}
+        want_wall = ref.wall()
+        ref.step(30)
+        want_next = ref.read("CurrPos")
+    runs = []
+    for ck in (30, 120, 500):  # 4 segments; one segment; one, with spare
+        with Stepper(s, p, device=0) as g:
+            hist = g.adjoint_forward(steps, ck)
+            for f in fields:
+                assert np.array_equal(g.read(f), want[f]), f"ck={ck}: {f} after the recorded pass"
+            w, r = g.wall()
+            assert np.array_equal(w, want_wall[0]) and np.array_equal(r, want_wall[1]), ck
+            bar = np.zeros_like(hist)
+            bar[:, 2] = 2.0 * hist[:, 2]
+            raw = g.adjoint_backward(bar)
+            assert np.array_equal(g.read("CurrPos"), want["CurrPos"]), f"ck={ck} after backward"
+            assert np.array_equal(g.read("vel"), want["vel"]), f"ck={ck} after backward"
+            g.step(30)
+            assert np.array_equal(g.read("CurrPos"), want_next), f"ck={ck}: stepping on"
+            runs.append((ck, hist, raw))
+    print([(ck, raw) for ck, _, raw in runs], flush=True)
+    for ck, hist, raw in runs[1:]:
+        assert np.array_equal(hist, runs[0][1]), ck
+        np.testing.assert_allclose(raw, runs[0][2], rtol=1e-12)
