@@ -163,18 +163,22 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
         }
         const double xn = sqrt(red<D>(xb, xb));
         const double n = sqrt(red<D>(eta, eta));
-        const double s = (n - xn) / xn;
+        // reciprocals by two Newton steps and products in place of the IEEE
+        // divisions: the adjoint keeps no reference rounding, and these are
+        // within an ulp or two for the bond lengths of any live bond
+        const double rxn = recip_nr(xn), rn = recip_nr(n);
+        const double s = (n - xn) * rxn;
         double eq = 0.0, el = 0.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-          e[m] = eta[m] / n;
+          e[m] = eta[m] * rn;
           q[m] = c * Vi * lFj[m] - c * Vj * lF[m];
           eq += e[m] * q[m];
           el += e[m] * lF[m];
         }
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-          const double du = e[m] * eq / xn + s * (q[m] - e[m] * eq) / n;
+          const double du = e[m] * eq * rxn + s * (q[m] - e[m] * eq) * rn;
           au[m] += live ? du : 0.0;
         }
         const double dc = c * Vj * s * el;
