@@ -96,8 +96,10 @@ __device__ __forceinline__ bool pair_gate(const AdjArgs &B, uint32_t t, const Gr
 // cotangents are summed with shuffles and lane 0 writes the node's result.
 // The adjoint has no reference order to keep, so the sums run in any fixed
 // order.
+constexpr uint32_t ADJ_BLOCK = 256; // threads per workgroup of k_adj_step
+
 template <int D>
-__global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_t t) {
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_step(const AdjArgs B, const uint32_t t) {
   constexpr int N = 2 * D + 2; // dual directions: x_i (D), v_i (D), K, mu
   constexpr uint32_t L = ADJ_LANES;
   static_assert(L % BGROUP == 0 && (L & (L - 1)) == 0, "whole column groups per pass");
@@ -340,21 +342,26 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs B, const uint32_
     for (int k = 0; k < D; ++k) B.lf_out[(size_t)k * ns + i] = nlF[k];
   }
 
-  // deterministic block sums of the parameter partials
-  __shared__ double red_s[3][256];
-  red_s[0][threadIdx.x] = gc;
-  red_s[1][threadIdx.x] = gk;
-  red_s[2][threadIdx.x] = gm;
-  __syncthreads();
-  for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off)
+  // deterministic block sums of the parameter partials: each wavefront's by
+  // shuffles, then the four wavefronts' in a fixed order
+  static_assert(ADJ_BLOCK == 256, "four wavefronts per workgroup");
 #pragma unroll
-      for (int q = 0; q < 3; ++q) red_s[q][threadIdx.x] += red_s[q][threadIdx.x + off];
-    __syncthreads();
+  for (int off = 32; off >= 1; off >>= 1) {
+    gc += __shfl_xor(gc, off);
+    gk += __shfl_xor(gk, off);
+    gm += __shfl_xor(gm, off);
   }
+  __shared__ double red_s[3][4];
+  if ((threadIdx.x & 63) == 0) {
+    red_s[0][threadIdx.x >> 6] = gc;
+    red_s[1][threadIdx.x >> 6] = gk;
+    red_s[2][threadIdx.x >> 6] = gm;
+  }
+  __syncthreads();
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) B.gpart[(size_t)blockIdx.x * 3 + q] += red_s[q][0];
+    for (int q = 0; q < 3; ++q)
+      B.gpart[(size_t)blockIdx.x * 3 + q] += (red_s[q][0] + red_s[q][1]) + (red_s[q][2] + red_s[q][3]);
 }
 
 // Largest displacement of a contact node over `steps` records of the segment
@@ -401,8 +408,8 @@ __global__ void k_adj_init_break(const uint32_t *bcol, uint32_t *break_step, uin
 
 hipError_t launch_adj_step(int dim, const AdjArgs &B, uint32_t t, uint32_t blocks,
                            hipStream_t s) {
-  if (dim == 2) hipLaunchKernelGGL(k_adj_step<2>, dim3(blocks), dim3(256), 0, s, B, t);
-  else hipLaunchKernelGGL(k_adj_step<3>, dim3(blocks), dim3(256), 0, s, B, t);
+  if (dim == 2) hipLaunchKernelGGL(k_adj_step<2>, dim3(blocks), dim3(ADJ_BLOCK), 0, s, B, t);
+  else hipLaunchKernelGGL(k_adj_step<3>, dim3(blocks), dim3(ADJ_BLOCK), 0, s, B, t);
   return hipGetLastError();
 }
 
