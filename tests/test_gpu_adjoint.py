@@ -240,18 +240,20 @@ def test_random_adjoint_cases_match_cpu_dual_numbers(case, oracle_threads):
     assert np.all(np.abs(raw - raw_cpu) <= 1e-8 * mag), (raw, raw_cpu, mag)
 
 
-def test_stored_trajectory_equals_checkpointed_recompute():
+@pytest.mark.parametrize("pre", [0, 7])
+def test_stored_trajectory_equals_checkpointed_recompute(pre):
     # One segment (checkpoint_every >= nsteps) keeps every step's state from
     # the recorded pass; several segments recompute them from checkpoints.
     # Both give the same reaction history and gradient (the latter to the
     # order in which the contact terms of different candidate lists are
-    # summed), and the handle reads and steps on like a plain run.
+    # summed), and the handle reads and steps on like a plain run.  pre:
+    # plain steps before the recorded pass.
     from perigrain_amd import Stepper
     s, p = _fracturing_compression()
     steps = 120
     fields = ("CurrPos", "vel", "acc", "force")
     with Stepper(s, p, device=0) as ref:
-        ref.step(steps)
+        ref.step(pre + steps)
         want = {f: ref.read(f) for f in fields}
         want_wall = ref.wall()
         ref.step(30)
@@ -259,6 +261,8 @@ def test_stored_trajectory_equals_checkpointed_recompute():
     runs = []
     for ck in (30, 120, 500):  # 4 segments; one segment; one, with spare
         with Stepper(s, p, device=0) as g:
+            if pre:
+                g.step(pre)
             hist = g.adjoint_forward(steps, ck)
             for f in fields:
                 assert np.array_equal(g.read(f), want[f]), f"ck={ck}: {f} after the recorded pass"
