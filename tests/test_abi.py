@@ -85,3 +85,17 @@ def test_null_handle_is_an_error_without_a_device():
     assert lib.pg_step(None, 1) == invalid
     assert b"null handle" in lib.pg_last_error()
     assert lib.pg_sync(None) == invalid
+
+
+def test_adjoint_keeps_the_whole_run_when_it_fits():
+    # diffrun's default checkpoint interval: the whole run (no recompute in
+    # the backward pass) while every step's state fits the budget, else
+    # checkpoints every ceil(sqrt(steps)) steps
+    from perigrain_amd import diffrun, setups
+    s, _ = setups.compression_2d_diff(n_grains=4, nodes_per_grain=60)
+    per_step = diffrun.trajectory_bytes(s, 1)
+    ns = (s.n_nodes + 1 + 63) // 64 * 64
+    assert per_step == ns * 24 * s.dim + s.n_grains * 2 * s.dim * 8 + 48
+    assert diffrun.auto_checkpoint_every(s, 2000) == 2000
+    assert diffrun.auto_checkpoint_every(s, 2000, budget=per_step * 1999) == 45
+    assert diffrun.auto_checkpoint_every(s, 2000, budget=per_step * 2000) == 2000
