@@ -62,6 +62,12 @@ def _draw(case):
     for k in rng.choice(len(pool), size=int(rng.integers(0, 4)), replace=False):
         pool[k](s, p)
         names.append(pool[k].__name__.strip("_"))
+    # one case in four follows run_timeloop_compact or run_timeloop_worm
+    # (drawn from a stream of its own, so the draws above stay as they were)
+    loop = int(np.random.default_rng(2000 + case).choice([0, 0, 0, 0, 0, 0, 2, 1]))
+    if loop and np.all(np.diff(s.node_offset) >= 4):
+        p.timeloop_variant = loop
+        names.append(("", "worm-loop", "compact-loop")[loop])
     return kind, s, p, names
 
 
