@@ -59,8 +59,11 @@ def _same_setup(a, b):
         assert getattr(a, name) == getattr(b, name), name
 
 
+@pytest.mark.parametrize("int_dtype", [np.int64, np.int32, np.uint32])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_all_h5_round_trip(tmp_path, dim):
+def test_all_h5_round_trip(tmp_path, dim, int_dtype):
+    # int64 is what h5py writes from numpy's default integers (exp_dict.py);
+    # int32 and uint32 files load the same through HDF5's conversions
     if dim == 2:
         s, _ = setups.two_disk_collision(seed=3)
     else:
@@ -72,7 +75,7 @@ def test_all_h5_round_trip(tmp_path, dim):
     s.clamped_offset[1:] = 3
     s.clamped_offset[0] = 0
     path = str(tmp_path / "all.h5")
-    h5io.write_all_h5(path, s)
+    h5io.write_all_h5(path, s, int_dtype=int_dtype)
     assert h5io.dataset_shape(path, "P_00001/Pos") == (int(s.node_offset[1]), dim)
     assert h5io.dataset_shape(path, "P_00001/clamped_nodes") == (3,)
     back, p = h5io.load_setup(path, dim)
